@@ -1,0 +1,136 @@
+// mtk::shgemm — SHGEMM / HSGEMM for AMD Instinct MI355X (gfx950).
+//
+// Drop-in C++ interface for the API declared in the reference's
+// include/shgemm/shgemm.hpp:1-159.  Names, enum values, argument order and
+// argument meaning are kept; `set_cuda_stream` takes a hipStream_t.
+//
+//   C = alpha * op(A) * op(B) + beta * C          (column-major, BLAS style)
+//   shgemm: A is FP32, B is FP16                   (reference shgemm.hpp:105-119)
+//   hsgemm: A is FP16, B is FP32                   (reference shgemm.hpp:121-155)
+//
+// op_t on an operand means it is stored transposed (A: k x m, lda >= k).
+// op_c == op_t stores C transposed: C is n x m, ldc >= n.
+// alpha and beta are host pointers read at call time (reference shgemm.cu:532,600).
+// A, B and C are device pointers owned by the caller.  All work is enqueued on
+// handle.cuda_stream; nothing synchronises.
+//
+// Error behaviour: the reference validates nothing.  This library checks its
+// arguments and throws mtk::shgemm::error (a std::runtime_error) before any
+// work is enqueued; launch failures are reported the same way.
+#ifndef SHGEMM_AMD_SHGEMM_HPP
+#define SHGEMM_AMD_SHGEMM_HPP
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+namespace mtk {
+namespace shgemm {
+
+class error : public std::runtime_error {
+ public:
+  explicit error(const std::string& what) : std::runtime_error(what) {}
+};
+
+namespace detail {
+// reference shgemm.hpp:30-34
+enum kernel_level {
+  P0 = 0,  // small tiles
+  P1 = 1,  // large tiles
+  num_levels
+};
+struct handle_state;  // kernel table, workspace, device properties
+}  // namespace detail
+
+// reference shgemm.hpp:36-39
+enum operation_t { op_n, op_t };
+
+// reference shgemm.hpp:41-44.  gfx950 has no TF32 matrix instruction:
+// tf32 selects a BF16 split (A in three BF16 parts, B in two) which keeps the
+// FP32 exponent range; fp16 selects the FP16 hi + 2^-11-scaled lo split.
+enum tc_t { fp16, tf32 };
+
+// reference shgemm.hpp:46-92.  The reference keeps 48 kernel descriptors in
+// the struct; here they live behind `state`.  The public fields a caller may
+// touch keep their names (including the reference's spelling of
+// fixed_lernel_level).
+struct shgemmHandle_t {
+  hipStream_t cuda_stream = nullptr;
+  unsigned debug_mode = 0;
+  detail::kernel_level fixed_lernel_level = detail::num_levels;
+  detail::handle_state* state = nullptr;
+};
+
+// reference shgemm.hpp:94-102
+void create(shgemmHandle_t& handle);
+void destroy(shgemmHandle_t& handle);
+void enable_kernel_level_fixing(shgemmHandle_t& handle, const detail::kernel_level kernel_level);
+void disable_kernel_level_fixing(shgemmHandle_t& handle);
+void set_debug_mode(shgemmHandle_t& handle, const unsigned on);
+void set_cuda_stream(shgemmHandle_t& handle, hipStream_t const cuda_stream);
+
+// reference shgemm.hpp:105-119
+detail::kernel_level shgemm(
+    const shgemmHandle_t handle,
+    const operation_t op_a,
+    const operation_t op_b,
+    const std::size_t m,
+    const std::size_t n,
+    const std::size_t k,
+    const float* const alpha_ptr,
+    const float* const a_ptr, const std::size_t lda,
+    const half* const b_ptr, const std::size_t ldb,
+    const float* const beta_ptr,
+    float* const c_ptr, const std::size_t ldc,
+    const tc_t compute_type,
+    const operation_t op_c = op_n);
+
+// reference shgemm.hpp:121-155: C = op(A16) op(B32) is computed as
+// C^T = op'(B32) op'(A16) by swapping the operands and inverting every op.
+inline detail::kernel_level hsgemm(
+    const shgemmHandle_t handle,
+    const operation_t op_a,
+    const operation_t op_b,
+    const std::size_t m,
+    const std::size_t n,
+    const std::size_t k,
+    const float* const alpha_ptr,
+    const half* const a_ptr, const std::size_t lda,
+    const float* const b_ptr, const std::size_t ldb,
+    const float* const beta_ptr,
+    float* const c_ptr, const std::size_t ldc,
+    const tc_t compute_type,
+    const operation_t op_c = op_n) {
+  const auto flip = [](const operation_t op) { return op == op_n ? op_t : op_n; };
+  return shgemm(handle, flip(op_b), flip(op_a), n, m, k, alpha_ptr, b_ptr, ldb, a_ptr, lda,
+                beta_ptr, c_ptr, ldc, compute_type, flip(op_c));
+}
+
+// ---------------------------------------------------------------------------
+// Additions with no counterpart in the reference.
+// ---------------------------------------------------------------------------
+
+// Plain FP32 SGEMM on FP32 MFMA (v_mfma_f32_32x32x2_f32): the comparator the
+// error-corrected SHGEMM is measured against (same layout rules as shgemm).
+void sgemm_fp32(
+    const shgemmHandle_t handle,
+    const operation_t op_a, const operation_t op_b,
+    const std::size_t m, const std::size_t n, const std::size_t k,
+    const float* const alpha_ptr,
+    const float* const a_ptr, const std::size_t lda,
+    const float* const b_ptr, const std::size_t ldb,
+    const float* const beta_ptr,
+    float* const c_ptr, const std::size_t ldc,
+    const operation_t op_c = op_n);
+
+// Per-row power-of-two prescaling of the FP32 operand for the fp16 compute
+// type.  Off by default (the reference's fp16 mode has FP16's exponent range).
+// When on, one extra pass over A computes each row's exponent; the split runs
+// on A * 2^-e(row) and the row of C is scaled back exactly.
+void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
+
+}  // namespace shgemm
+}  // namespace mtk
+#endif

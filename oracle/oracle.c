@@ -1,0 +1,276 @@
+/* Test infrastructure only: the CPU oracle for the SHGEMM hot path.
+ *
+ * Nothing in the product links, loads or calls this file.  Tests, the smoke
+ * check and the CPU-baseline leg of the benchmark use it as the checker.
+ *
+ *   oracle_gemm_f64  C64 = alpha * op(A) * op(B) + beta * C0, FP64 accumulation
+ *                    of FP32 inputs (B given as float of its FP16 value, as the
+ *                    reference's tests do: test/shgemm_test.cu:214-224, 91-100).
+ *                    Replaces mateval::get_error_AxB's reference product
+ *                    (un-vendored submodule test/mateval, .gitmodules:7-9).
+ *   oracle_gemm_f32  the same product with FP32 accumulation: the host SGEMM
+ *                    that is timed as the CPU baseline.
+ *   oracle_split_f16 hi = fp16(a), lo = fp16((a - hi) * 2^11), both
+ *                    round-to-nearest-even (reference src/wmmae_shgemm.hpp:29-31;
+ *                    method of Ootomo & Yokota, IJHPCA 2022).
+ *   oracle_split_bf16x3 / oracle_split_bf16x2  the BF16 parts of the tf32
+ *                    compute type.
+ *
+ * Conversions are written from the IEEE 754 binary16/bfloat16 definitions
+ * (round to nearest, ties to even; overflow to infinity; gradual underflow).
+ * Layout: column-major; op_t means the operand is stored transposed.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- */
+/* binary32 -> binary16, round to nearest even                       */
+/* ---------------------------------------------------------------- */
+uint16_t oracle_f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ex = (x >> 23) & 0xffu;
+  uint32_t man = x & 0x7fffffu;
+  if (ex == 0xffu) { /* inf / nan */
+    if (man == 0) return (uint16_t)(sign | 0x7c00u);
+    return (uint16_t)(sign | 0x7e00u | (man >> 13));
+  }
+  const int e = (int)ex - 127; /* unbiased */
+  if (e > 15) return (uint16_t)(sign | 0x7c00u);
+  if (e >= -14) {
+    /* normal half: 10 mantissa bits kept, 13 dropped */
+    uint32_t h = ((uint32_t)(e + 15) << 10) | (man >> 13);
+    const uint32_t rest = man & 0x1fffu;
+    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) h += 1; /* carries into exponent/inf correctly */
+    return (uint16_t)(sign | h);
+  }
+  /* subnormal half (or zero): value = m * 2^-24 */
+  if (ex == 0) return (uint16_t)sign; /* float subnormals are far below half's range */
+  man |= 0x800000u;                  /* implicit bit; value = man * 2^(e-23) */
+  const int shift = -e - 1;          /* bits to drop: man * 2^(e-23) / 2^-24 = man >> (-(e+1)) */
+  if (shift > 24) return (uint16_t)sign;
+  uint32_t h = man >> shift;
+  const uint32_t rest = man & ((1u << shift) - 1u);
+  const uint32_t half = 1u << (shift - 1);
+  if (rest > half || (rest == half && (h & 1u))) h += 1;
+  return (uint16_t)(sign | h);
+}
+
+float oracle_f16_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t ex = (h >> 10) & 0x1fu;
+  const uint32_t man = h & 0x3ffu;
+  float f;
+  if (ex == 0x1fu) {
+    const uint32_t x = sign | 0x7f800000u | (man << 13);
+    memcpy(&f, &x, 4);
+    return f;
+  }
+  if (ex == 0) {
+    f = ldexpf((float)man, -24);
+    return sign ? -f : f;
+  }
+  const uint32_t x = sign | ((ex + 112u) << 23) | (man << 13);
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+/* binary32 -> bfloat16, round to nearest even */
+uint16_t oracle_f32_to_bf16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7f800000u) == 0x7f800000u && (x & 0x7fffffu)) return (uint16_t)((x >> 16) | 0x40u); /* quiet nan */
+  const uint32_t rest = x & 0xffffu;
+  uint32_t hi = x >> 16;
+  if (rest > 0x8000u || (rest == 0x8000u && (hi & 1u))) hi += 1;
+  return (uint16_t)hi;
+}
+
+float oracle_bf16_to_f32(uint16_t b) {
+  const uint32_t x = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+void oracle_split_f16(const float* a, uint16_t* hi, uint16_t* lo, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const uint16_t h = oracle_f32_to_f16(a[i]);
+    const float r = a[i] - oracle_f16_to_f32(h); /* exact */
+    hi[i] = h;
+    lo[i] = oracle_f32_to_f16(r * 2048.0f);
+  }
+}
+
+void oracle_split_bf16x3(const float* a, uint16_t* p0, uint16_t* p1, uint16_t* p2, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const uint16_t b0 = oracle_f32_to_bf16(a[i]);
+    const float r1 = a[i] - oracle_bf16_to_f32(b0);
+    const uint16_t b1 = oracle_f32_to_bf16(r1);
+    const float r2 = r1 - oracle_bf16_to_f32(b1);
+    p0[i] = b0;
+    p1[i] = b1;
+    p2[i] = oracle_f32_to_bf16(r2);
+  }
+}
+
+void oracle_split_bf16x2(const uint16_t* b16, uint16_t* p0, uint16_t* p1, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float f = oracle_f16_to_f32(b16[i]);
+    const uint16_t b0 = oracle_f32_to_bf16(f);
+    p0[i] = b0;
+    p1[i] = oracle_f32_to_bf16(f - oracle_bf16_to_f32(b0));
+  }
+}
+
+/* ---------------------------------------------------------------- */
+/* GEMM                                                              */
+/* ---------------------------------------------------------------- */
+/* element (i, p) of op(A) (m x k) and (p, j) of op(B) (k x n) */
+static inline float a_at(const float* A, int64_t lda, int op_a, int64_t i, int64_t p) {
+  return op_a == 0 ? A[i + p * lda] : A[p + i * lda];
+}
+static inline float b_at(const float* B, int64_t ldb, int op_b, int64_t p, int64_t j) {
+  return op_b == 0 ? B[p + j * ldb] : B[j + p * ldb];
+}
+static inline int64_t c_off(int op_c, int64_t i, int64_t j, int64_t ldc) {
+  return op_c == 0 ? i + j * ldc : j + i * ldc;
+}
+
+#define KB 256
+#define JB 256
+#define IB 32
+
+/* rows [r0, r1) of op(A) only: the row-sampled oracle for large shapes.
+ * out (FP64) uses C's layout (op_c, ldc).  C0 may be NULL when beta == 0. */
+void oracle_gemm_f64_rows(int op_a, int op_b, int op_c, int64_t m, int64_t n, int64_t k, double alpha,
+                          const float* A, int64_t lda, const float* B, int64_t ldb, double beta, const float* C0,
+                          int64_t ldc, double* out, int64_t r0, int64_t r1, int nthreads) {
+  (void)m;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const int64_t nrows = r1 - r0;
+  const int64_t nib = (nrows + IB - 1) / IB;
+#pragma omp parallel
+  {
+    double* acc = (double*)malloc(sizeof(double) * IB * JB);
+    float* bp = (float*)malloc(sizeof(float) * KB * JB);
+    float* ap = (float*)malloc(sizeof(float) * IB * KB);
+#pragma omp for schedule(dynamic) collapse(2)
+    for (int64_t jb = 0; jb < (n + JB - 1) / JB; ++jb) {
+      for (int64_t ib = 0; ib < nib; ++ib) {
+        const int64_t j0 = jb * JB, j1 = (j0 + JB < n) ? j0 + JB : n, nj = j1 - j0;
+        const int64_t i0 = r0 + ib * IB, i1 = (i0 + IB < r1) ? i0 + IB : r1, ni = i1 - i0;
+        memset(acc, 0, sizeof(double) * IB * JB);
+        for (int64_t p0 = 0; p0 < k; p0 += KB) {
+          const int64_t p1 = (p0 + KB < k) ? p0 + KB : k, np = p1 - p0;
+          for (int64_t p = 0; p < np; ++p)
+            for (int64_t j = 0; j < nj; ++j) bp[p * JB + j] = b_at(B, ldb, op_b, p0 + p, j0 + j);
+          for (int64_t i = 0; i < ni; ++i)
+            for (int64_t p = 0; p < np; ++p) ap[i * KB + p] = a_at(A, lda, op_a, i0 + i, p0 + p);
+          for (int64_t i = 0; i < ni; ++i) {
+            double* ci = acc + i * JB;
+            for (int64_t p = 0; p < np; ++p) {
+              const double a = (double)ap[i * KB + p];
+              const float* bj = bp + p * JB;
+              for (int64_t j = 0; j < nj; ++j) ci[j] += a * (double)bj[j];
+            }
+          }
+        }
+        for (int64_t i = 0; i < ni; ++i)
+          for (int64_t j = 0; j < nj; ++j) {
+            const int64_t o = c_off(op_c, i0 + i, j0 + j, ldc);
+            double v = alpha * acc[i * JB + j];
+            if (beta != 0.0) v += beta * (double)C0[o];
+            out[o] = v;
+          }
+      }
+    }
+    free(acc);
+    free(bp);
+    free(ap);
+  }
+}
+
+void oracle_gemm_f64(int op_a, int op_b, int op_c, int64_t m, int64_t n, int64_t k, double alpha, const float* A,
+                     int64_t lda, const float* B, int64_t ldb, double beta, const float* C0, int64_t ldc,
+                     double* out, int nthreads) {
+  oracle_gemm_f64_rows(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, out, 0, m, nthreads);
+}
+
+/* Host FP32 GEMM: same blocking, FP32 accumulation (each multiply-add rounded
+ * to FP32; the compiler fuses them into FMAs), rows [r0, r1). */
+void oracle_gemm_f32_rows(int op_a, int op_b, int op_c, int64_t m, int64_t n, int64_t k, float alpha,
+                          const float* A, int64_t lda, const float* B, int64_t ldb, float beta, const float* C0,
+                          int64_t ldc, float* out, int64_t r0, int64_t r1, int nthreads) {
+  (void)m;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const int64_t nrows = r1 - r0;
+  const int64_t nib = (nrows + IB - 1) / IB;
+#pragma omp parallel
+  {
+    float* acc = (float*)malloc(sizeof(float) * IB * JB);
+    float* bp = (float*)malloc(sizeof(float) * KB * JB);
+    float* ap = (float*)malloc(sizeof(float) * IB * KB);
+#pragma omp for schedule(dynamic) collapse(2)
+    for (int64_t jb = 0; jb < (n + JB - 1) / JB; ++jb) {
+      for (int64_t ib = 0; ib < nib; ++ib) {
+        const int64_t j0 = jb * JB, j1 = (j0 + JB < n) ? j0 + JB : n, nj = j1 - j0;
+        const int64_t i0 = r0 + ib * IB, i1 = (i0 + IB < r1) ? i0 + IB : r1, ni = i1 - i0;
+        memset(acc, 0, sizeof(float) * IB * JB);
+        for (int64_t p0 = 0; p0 < k; p0 += KB) {
+          const int64_t p1 = (p0 + KB < k) ? p0 + KB : k, np = p1 - p0;
+          for (int64_t p = 0; p < np; ++p)
+            for (int64_t j = 0; j < nj; ++j) bp[p * JB + j] = b_at(B, ldb, op_b, p0 + p, j0 + j);
+          for (int64_t i = 0; i < ni; ++i)
+            for (int64_t p = 0; p < np; ++p) ap[i * KB + p] = a_at(A, lda, op_a, i0 + i, p0 + p);
+          for (int64_t i = 0; i < ni; ++i) {
+            float* ci = acc + i * JB;
+            for (int64_t p = 0; p < np; ++p) {
+              const float a = ap[i * KB + p];
+              const float* bj = bp + p * JB;
+              for (int64_t j = 0; j < nj; ++j) ci[j] += a * bj[j];
+            }
+          }
+        }
+        for (int64_t i = 0; i < ni; ++i)
+          for (int64_t j = 0; j < nj; ++j) {
+            const int64_t o = c_off(op_c, i0 + i, j0 + j, ldc);
+            float v = alpha * acc[i * JB + j];
+            if (beta != 0.0f) v += beta * C0[o];
+            out[o] = v;
+          }
+      }
+    }
+    free(acc);
+    free(bp);
+    free(ap);
+  }
+}
+
+void oracle_gemm_f32(int op_a, int op_b, int op_c, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                     int64_t lda, const float* B, int64_t ldb, float beta, const float* C0, int64_t ldc, float* out,
+                     int nthreads) {
+  oracle_gemm_f32_rows(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, out, 0, m, nthreads);
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

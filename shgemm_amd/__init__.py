@@ -1,0 +1,196 @@
+"""shgemm_amd — Python binding of the MI355X SHGEMM library (C ABI via ctypes).
+
+Mirrors the reference's mtk::shgemm interface (include/shgemm/shgemm.hpp:94-155):
+create / destroy / set_cuda_stream / enable_kernel_level_fixing /
+disable_kernel_level_fixing / set_debug_mode / shgemm / hsgemm, with the same
+argument order and meaning.  Device operands are HIP device pointers given as
+ints or as torch tensors on a ROCm device (their data_ptr() is used); alpha and
+beta are Python floats.  Failures raise ShgemmError.
+
+The library is libshgemm_amd.so next to this file (built by
+shgemm_amd/build.py).  There is no fallback: importing works without it, but
+every call raises if the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import re
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libshgemm_amd.so"
+HEADER_PATH = _HERE.parent / "include" / "shgemm_amd.h"
+
+_lib = None
+
+
+class ShgemmError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"shgemm_amd error {status}: {message}")
+        self.status = status
+
+
+class operation_t(enum.IntEnum):  # reference include/shgemm/shgemm.hpp:36-39
+    op_n = 0
+    op_t = 1
+
+
+class tc_t(enum.IntEnum):  # reference include/shgemm/shgemm.hpp:41-44
+    fp16 = 0
+    tf32 = 1
+
+
+class kernel_level(enum.IntEnum):  # reference include/shgemm/shgemm.hpp:30-34
+    P0 = 0
+    P1 = 1
+
+
+op_n, op_t = operation_t.op_n, operation_t.op_t
+fp16, tf32 = tc_t.fp16, tc_t.tf32
+STATUS_OK, STATUS_EINVAL, STATUS_EHIP, STATUS_EINTERNAL = 0, -1, -2, -3
+
+
+def exported_symbols_in_header() -> list[str]:
+    """Names of every function declared in include/shgemm_amd.h."""
+    text = HEADER_PATH.read_text()
+    return sorted(set(re.findall(r"\b(shgemm_amd_[a-z0-9_]+)\s*\(", text)))
+
+
+def load_library(path: Path | str | None = None) -> ctypes.CDLL:
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise ShgemmError(STATUS_EINTERNAL, f"native library {p} is missing; run shgemm_amd/build.py")
+    L = ctypes.CDLL(str(p))
+    vp, sz, i32, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.shgemm_amd_create.argtypes = [ctypes.POINTER(vp)]
+    L.shgemm_amd_destroy.argtypes = [vp]
+    L.shgemm_amd_set_stream.argtypes = [vp, vp]
+    L.shgemm_amd_enable_kernel_level_fixing.argtypes = [vp, i32]
+    L.shgemm_amd_disable_kernel_level_fixing.argtypes = [vp]
+    L.shgemm_amd_set_debug_mode.argtypes = [vp, u32]
+    L.shgemm_amd_shgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
+    L.shgemm_amd_hsgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
+    L.shgemm_amd_sgemm_fp32.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32]
+    L.shgemm_amd_set_exponent_scaling.argtypes = [vp, i32]
+    L.shgemm_amd_last_error.restype = ctypes.c_char_p
+    L.shgemm_amd_version.restype = ctypes.c_char_p
+    for name in exported_symbols_in_header():
+        if name not in ("shgemm_amd_last_error", "shgemm_amd_version"):
+            getattr(L, name).restype = i32
+    if path is None:
+        _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != STATUS_OK:
+        msg = load_library().shgemm_amd_last_error().decode(errors="replace")
+        raise ShgemmError(status, msg)
+
+
+def _dptr(x) -> int | None:
+    """Device address of a torch tensor or an int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if hasattr(x, "is_cuda") and not x.is_cuda:
+            raise ShgemmError(STATUS_EINVAL, "operand tensor is not on the GPU")
+        return int(x.data_ptr())
+    raise TypeError(f"expected a device tensor or an int address, got {type(x)}")
+
+
+def _fptr(v: float):
+    return ctypes.pointer(ctypes.c_float(float(v)))
+
+
+class shgemmHandle_t:
+    """Owns one native handle (reference shgemmHandle_t, shgemm.hpp:46-92)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        self.cuda_stream = None
+
+    @property
+    def ptr(self):
+        if not self._h:
+            raise ShgemmError(STATUS_EINVAL, "handle not created")
+        return self._h
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        destroy(self)
+
+
+def create(handle: shgemmHandle_t | None = None) -> shgemmHandle_t:
+    h = handle or shgemmHandle_t()
+    _check(load_library().shgemm_amd_create(ctypes.byref(h._h)))
+    return h
+
+
+def destroy(handle: shgemmHandle_t) -> None:
+    if handle._h:
+        _check(load_library().shgemm_amd_destroy(handle._h))
+        handle._h = ctypes.c_void_p()
+
+
+def set_cuda_stream(handle: shgemmHandle_t, stream) -> None:
+    """stream: a hipStream_t as int, a torch.cuda.Stream, or None (default)."""
+    raw = stream
+    if stream is not None and hasattr(stream, "cuda_stream"):
+        raw = stream.cuda_stream
+    _check(load_library().shgemm_amd_set_stream(handle.ptr, raw))
+    handle.cuda_stream = stream
+
+
+def enable_kernel_level_fixing(handle: shgemmHandle_t, level: int) -> None:
+    _check(load_library().shgemm_amd_enable_kernel_level_fixing(handle.ptr, int(level)))
+
+
+def disable_kernel_level_fixing(handle: shgemmHandle_t) -> None:
+    _check(load_library().shgemm_amd_disable_kernel_level_fixing(handle.ptr))
+
+
+def set_debug_mode(handle: shgemmHandle_t, on: int) -> None:
+    _check(load_library().shgemm_amd_set_debug_mode(handle.ptr, int(on)))
+
+
+def set_exponent_scaling(handle: shgemmHandle_t, on: bool) -> None:
+    _check(load_library().shgemm_amd_set_exponent_scaling(handle.ptr, 1 if on else 0))
+
+
+def shgemm(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, compute_type, op_c=op_n) -> kernel_level:
+    """C = alpha op(A32) op(B16) + beta C (reference shgemm.hpp:105-119)."""
+    lvl = ctypes.c_int(-1)
+    _check(load_library().shgemm_amd_shgemm(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha), _dptr(a), lda,
+                                            _dptr(b), ldb, _fptr(beta), _dptr(c), ldc, int(compute_type), int(op_c),
+                                            ctypes.byref(lvl)))
+    return kernel_level(lvl.value)
+
+
+def hsgemm(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, compute_type, op_c=op_n) -> kernel_level:
+    """C = alpha op(A16) op(B32) + beta C (reference shgemm.hpp:121-155)."""
+    lvl = ctypes.c_int(-1)
+    _check(load_library().shgemm_amd_hsgemm(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha), _dptr(a), lda,
+                                            _dptr(b), ldb, _fptr(beta), _dptr(c), ldc, int(compute_type), int(op_c),
+                                            ctypes.byref(lvl)))
+    return kernel_level(lvl.value)
+
+
+def sgemm_fp32(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, op_c=op_n) -> None:
+    """Plain FP32 SGEMM (the comparator), same conventions."""
+    _check(load_library().shgemm_amd_sgemm_fp32(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha), _dptr(a),
+                                                lda, _dptr(b), ldb, _fptr(beta), _dptr(c), ldc, int(op_c)))
+
+
+def version() -> str:
+    return load_library().shgemm_amd_version().decode()

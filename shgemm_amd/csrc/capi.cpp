@@ -1,0 +1,157 @@
+// C ABI over the mtk::shgemm C++ API (declared in include/shgemm_amd.h).
+// Exceptions never cross this boundary: each entry point returns a status and
+// records the message for shgemm_amd_last_error().
+#include <new>
+#include <string>
+
+#include "../../include/shgemm/shgemm.hpp"
+#include "../../include/shgemm_amd.h"
+#include "errors.hpp"
+
+struct shgemm_amd_handle {
+  mtk::shgemm::shgemmHandle_t h;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SHGEMM_AMD_OK;
+  } catch (const shgemm_amd::hip_failure& e) {
+    g_last_error = e.what();
+    return SHGEMM_AMD_EHIP;
+  } catch (const mtk::shgemm::error& e) {
+    g_last_error = e.what();
+    return SHGEMM_AMD_EINVAL;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SHGEMM_AMD_EINTERNAL;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SHGEMM_AMD_EINTERNAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return SHGEMM_AMD_EINTERNAL;
+  }
+}
+
+mtk::shgemm::operation_t to_op(int op) {
+  if (op == SHGEMM_AMD_OP_N) return mtk::shgemm::op_n;
+  if (op == SHGEMM_AMD_OP_T) return mtk::shgemm::op_t;
+  throw mtk::shgemm::error("invalid operation (expected 0 = N or 1 = T)");
+}
+
+mtk::shgemm::tc_t to_tc(int t) {
+  if (t == SHGEMM_AMD_FP16) return mtk::shgemm::fp16;
+  if (t == SHGEMM_AMD_TF32) return mtk::shgemm::tf32;
+  throw mtk::shgemm::error("invalid compute type (expected 0 = fp16 or 1 = tf32)");
+}
+
+void need(const shgemm_amd_handle* h) {
+  if (h == nullptr) throw mtk::shgemm::error("handle is null");
+}
+}  // namespace
+
+extern "C" {
+
+int shgemm_amd_create(shgemm_amd_handle** out) {
+  return guarded([&] {
+    if (out == nullptr) throw mtk::shgemm::error("out is null");
+    auto* h = new shgemm_amd_handle();
+    try {
+      mtk::shgemm::create(h->h);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int shgemm_amd_destroy(shgemm_amd_handle* h) {
+  return guarded([&] {
+    if (h == nullptr) return;
+    mtk::shgemm::destroy(h->h);
+    delete h;
+  });
+}
+
+int shgemm_amd_set_stream(shgemm_amd_handle* h, void* stream) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::set_cuda_stream(h->h, static_cast<hipStream_t>(stream));
+  });
+}
+
+int shgemm_amd_enable_kernel_level_fixing(shgemm_amd_handle* h, int level) {
+  return guarded([&] {
+    need(h);
+    if (level != 0 && level != 1) throw mtk::shgemm::error("kernel level must be 0 (P0) or 1 (P1)");
+    mtk::shgemm::enable_kernel_level_fixing(h->h, static_cast<mtk::shgemm::detail::kernel_level>(level));
+  });
+}
+
+int shgemm_amd_disable_kernel_level_fixing(shgemm_amd_handle* h) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::disable_kernel_level_fixing(h->h);
+  });
+}
+
+int shgemm_amd_set_debug_mode(shgemm_amd_handle* h, unsigned on) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::set_debug_mode(h->h, on);
+  });
+}
+
+int shgemm_amd_shgemm(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k, const float* alpha,
+                      const float* a, size_t lda, const uint16_t* b, size_t ldb, const float* beta, float* c,
+                      size_t ldc, int compute_type, int op_c, int* level_out) {
+  return guarded([&] {
+    need(h);
+    const auto lvl = mtk::shgemm::shgemm(h->h, to_op(op_a), to_op(op_b), m, n, k, alpha, a, lda,
+                                         reinterpret_cast<const half*>(b), ldb, beta, c, ldc, to_tc(compute_type),
+                                         to_op(op_c));
+    if (level_out) *level_out = static_cast<int>(lvl);
+  });
+}
+
+int shgemm_amd_hsgemm(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k, const float* alpha,
+                      const uint16_t* a, size_t lda, const float* b, size_t ldb, const float* beta, float* c,
+                      size_t ldc, int compute_type, int op_c, int* level_out) {
+  return guarded([&] {
+    need(h);
+    const auto lvl = mtk::shgemm::hsgemm(h->h, to_op(op_a), to_op(op_b), m, n, k, alpha,
+                                         reinterpret_cast<const half*>(a), lda, b, ldb, beta, c, ldc,
+                                         to_tc(compute_type), to_op(op_c));
+    if (level_out) *level_out = static_cast<int>(lvl);
+  });
+}
+
+int shgemm_amd_sgemm_fp32(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k,
+                          const float* alpha, const float* a, size_t lda, const float* b, size_t ldb,
+                          const float* beta, float* c, size_t ldc, int op_c) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::sgemm_fp32(h->h, to_op(op_a), to_op(op_b), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                            to_op(op_c));
+  });
+}
+
+int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::set_exponent_scaling(h->h, on != 0);
+  });
+}
+
+const char* shgemm_amd_last_error(void) { return g_last_error.c_str(); }
+
+const char* shgemm_amd_version(void) { return "shgemm_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Shared device-side definitions for the gfx950 GEMM kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace shgemm_amd {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+enum : int { OP_N = 0, OP_T = 1 };
+// MODE_FP16: A = hi + 2^-11 lo, both FP16 (reference src/wmmae_shgemm.hpp:29-31)
+// MODE_BF16: A = a0 + a1 + a2 and B = b0 + b1, all BF16 (the tf32 compute type)
+enum : int { MODE_FP16 = 0, MODE_BF16 = 1 };
+
+// Everything a GEMM launch needs.  Sizes are 32-bit (the host rejects larger
+// ones); leading dimensions and offsets are 64-bit.
+struct GemmParams {
+  int m, n, k;
+  float alpha, beta;
+  const float* a;      // FP32 operand
+  int64_t lda;
+  const void* b;       // FP16 bits (SHGEMM) or FP32 (SGEMM)
+  int64_t ldb;
+  float* c;
+  int64_t ldc;
+  float* ws;           // split-K partials, slice-major, each slice m*n floats
+  const int* row_exp;  // per-row exponent of A (nullptr: no prescaling)
+  int k_slice;         // k extent per slice (multiple of the tile depth)
+  int tiles_m, tiles_n;
+  int group_m;         // tile rows per L2 group
+  int vec_a, vec_b;    // 1 when base pointers and ld allow vector loads
+};
+
+// Workgroup id -> output tile.  Consecutive workgroup ids are dispatched
+// round-robin over the 8 XCDs; remap so each XCD owns a contiguous run of
+// tiles, then walk that run in groups of `group_m` tile rows so the tiles in
+// flight on one XCD share A and B panels in its L2.
+__device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& tm, int& tn) {
+  const int nb = p.tiles_m * p.tiles_n;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, i = bid >> 3;
+  const int t = x * q + (x < r ? x : r) + i;
+  const int per_group = p.group_m * p.tiles_n;
+  const int g = t / per_group;
+  const int first = g * p.group_m;
+  const int rows = min(p.tiles_m - first, p.group_m);
+  const int local = t - g * per_group;
+  tm = first + local % rows;
+  tn = local / rows;
+}
+
+// LDS operand layout: the k extent of a tile is cut into chunks of 16 bytes
+// (8 FP16/BF16 values, or 4 FP32 values).  Chunk (c, r) of an R-row operand
+// lives at slot c*R + (r ^ 4*(c&3)).  A wavefront reading 32 consecutive rows
+// of one chunk (the MFMA operand pattern) touches 32 distinct 16-byte slots in
+// each aligned group of 16 rows, and the writers' 4-row x 4-chunk patterns
+// also land in distinct bank groups.
+template <int R>
+__device__ __forceinline__ int lds_slot(int c, int r) {
+  return c * R + (r ^ ((c & 3) << 2));
+}
+
+// FP32 -> FP16 hi + 2^11-scaled FP16 lo, both round-to-nearest-even.
+// a - float(hi) is exact in FP32 (hi is the FP16 rounding of a).
+__device__ __forceinline__ void split_f16(float a, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)a;
+  const float r = a - (float)hi;
+  lo = (_Float16)(r * 2048.0f);
+}
+
+// FP32 -> three BF16 parts a0 + a1 + a2 (each residual exact in FP32).
+__device__ __forceinline__ void split_bf16x3(float a, __bf16& a0, __bf16& a1, __bf16& a2) {
+  a0 = (__bf16)a;
+  const float r1 = a - (float)a0;
+  a1 = (__bf16)r1;
+  const float r2 = r1 - (float)a1;
+  a2 = (__bf16)r2;
+}
+
+// FP16 -> two BF16 parts b0 + b1, exact (b has 11 significant bits).
+__device__ __forceinline__ void split_bf16x2(_Float16 b, __bf16& b0, __bf16& b1) {
+  const float f = (float)b;
+  b0 = (__bf16)f;
+  b1 = (__bf16)(f - (float)b0);
+}
+
+__device__ __forceinline__ _Float16 bits_to_f16(uint16_t u) {
+  return __builtin_bit_cast(_Float16, u);
+}
+
+__device__ __forceinline__ f32x16 mfma_f16(const f16x8& a, const f16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma_bf16(const bf16x8& a, const bf16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma_f32(float a, float b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+}  // namespace shgemm_amd

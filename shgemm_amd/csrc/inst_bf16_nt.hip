@@ -1,0 +1,5 @@
+// SHGEMM kernel variants: compute type bf16, op_a/op_b = nt.
+#include "register_shgemm.hpp"
+namespace shgemm_amd {
+void register_shgemm_bf16_nt(KernelTable& t) { put_shgemm_ops<OP_N, OP_T, MODE_BF16>(t); }
+}  // namespace shgemm_amd

@@ -1,0 +1,43 @@
+// Kernel table: every compiled kernel variant, filled by the instantiation
+// units (inst_*.hip) and read by the host dispatcher.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include "gemm_common.hpp"
+
+namespace shgemm_amd {
+
+struct KernelEntry {
+  const void* fn = nullptr;
+  size_t lds = 0;      // dynamic LDS bytes
+  int bm = 0, bn = 0, bk = 0;
+};
+
+enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };
+
+struct KernelTable {
+  // [op_a][op_b][op_c][mode][level][splitk][scaled]
+  KernelEntry shgemm[2][2][2][2][2][2][2];
+  // [op_a][op_b][op_c]
+  KernelEntry sgemm[2][2][2];
+};
+
+void register_shgemm_fp16_nn(KernelTable&);
+void register_shgemm_fp16_nt(KernelTable&);
+void register_shgemm_fp16_tn(KernelTable&);
+void register_shgemm_fp16_tt(KernelTable&);
+void register_shgemm_bf16_nn(KernelTable&);
+void register_shgemm_bf16_nt(KernelTable&);
+void register_shgemm_bf16_tn(KernelTable&);
+void register_shgemm_bf16_tt(KernelTable&);
+void register_sgemm(KernelTable&);
+void register_all(KernelTable&);
+
+// Auxiliary kernels (defined in aux_kernels.hip).
+hipError_t launch_splitk_reduce(const float* ws, int slices, int64_t elems, int inner, float alpha, float beta,
+                                float* c, int64_t ldc, hipStream_t s);
+hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_t ldc, hipStream_t s);
+hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
+                                int* row_exp, hipStream_t s);
+
+}  // namespace shgemm_amd

@@ -1,0 +1,343 @@
+// Host runtime of mtk::shgemm on HIP: handle life cycle, argument checks,
+// kernel choice (tile level / split-K) and launch.
+//
+// Reference counterpart: src/shgemm.cu:401-690 (create, destroy, stream and
+// level setters, the shgemm dispatcher).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/shgemm/shgemm.hpp"
+#include "errors.hpp"
+#include "registry.hpp"
+
+namespace mtk {
+namespace shgemm {
+namespace detail {
+
+struct handle_state {
+  int device = 0;
+  int num_cus = 0;
+  shgemm_amd::KernelTable table;
+  float* ws = nullptr;  // split-K partials
+  size_t ws_bytes = 0;
+  uint32_t* row_max = nullptr;  // prescale scratch
+  int* row_exp = nullptr;
+  size_t row_cap = 0;
+  bool exp_scaling = false;
+};
+
+}  // namespace detail
+
+namespace {
+
+// Split-K workspace: 64 MiB of FP32 partials (reference: 3.54 MB,
+// include/shgemm/shgemm.hpp:90, sized for one 108-SM GPU).
+constexpr size_t kWorkspaceBytes = size_t(64) << 20;
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw shgemm_amd::hip_failure(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const shgemm_amd::KernelTable& kernel_table() {
+  static shgemm_amd::KernelTable t;
+  static std::once_flag once;
+  std::call_once(once, [] { shgemm_amd::register_all(t); });
+  return t;
+}
+
+bool aligned(const void* p, size_t bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
+
+// Tile ordering: group this many tile rows so tiles in flight share panels.
+int group_rows(int tiles_m) { return std::min(tiles_m, 8); }
+
+void set_lds_limit(const shgemm_amd::KernelEntry& e) {
+  if (e.lds > 65536)
+    check_hip(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds),
+              "hipFuncSetAttribute");
+}
+
+void launch(const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmParams& p, hipStream_t s) {
+  if (e.fn == nullptr) throw error("internal: kernel variant not compiled");
+  void* args[] = {const_cast<shgemm_amd::GemmParams*>(&p)};
+  check_hip(hipLaunchKernel(e.fn, grid, dim3(256), args, e.lds, s), "kernel launch");
+}
+
+const char* op_name(operation_t op) { return op == op_n ? "N" : "T"; }
+
+struct Shape {
+  size_t m, n, k;
+};
+
+void validate_common(const shgemmHandle_t& handle, operation_t op_a, operation_t op_b, operation_t op_c, size_t m,
+                     size_t n, size_t k, const float* alpha, const void* a, size_t lda, const void* b, size_t ldb,
+                     const float* beta, const float* c, size_t ldc) {
+  if (handle.state == nullptr) throw error("handle not created (call mtk::shgemm::create first)");
+  if ((op_a != op_n && op_a != op_t) || (op_b != op_n && op_b != op_t) || (op_c != op_n && op_c != op_t))
+    throw error("invalid operation_t");
+  if (m > (size_t)INT_MAX || n > (size_t)INT_MAX || k > (size_t)INT_MAX)
+    throw error("m, n and k must each be below 2^31");
+  if (alpha == nullptr || beta == nullptr) throw error("alpha and beta must be host pointers, not null");
+  const size_t a_rows = op_a == op_n ? m : k, b_rows = op_b == op_n ? k : n, c_rows = op_c == op_n ? m : n;
+  if (lda < std::max<size_t>(1, a_rows)) throw error("lda < rows of the stored A");
+  if (ldb < std::max<size_t>(1, b_rows)) throw error("ldb < rows of the stored B");
+  if (ldc < std::max<size_t>(1, c_rows)) throw error("ldc < rows of the stored C");
+  if (m > 0 && n > 0) {
+    if (c == nullptr) throw error("C is null");
+    if (k > 0 && (a == nullptr || b == nullptr)) throw error("A or B is null");
+  }
+}
+
+}  // namespace
+
+void create(shgemmHandle_t& handle) {
+  handle.cuda_stream = nullptr;
+  handle.fixed_lernel_level = detail::num_levels;
+  handle.debug_mode = 0;
+  const char* dbg = std::getenv("MTK_SHGEMM_DEBUG");
+  if (dbg != nullptr && dbg[0] == '1') handle.debug_mode = 1;
+  const char* lvl = std::getenv("MTK_SHGEMM_KERNEL_LEVEL");
+  if (lvl != nullptr && (lvl[0] == '0' || lvl[0] == '1') && lvl[1] == '\0')
+    handle.fixed_lernel_level = static_cast<detail::kernel_level>(lvl[0] - '0');
+
+  auto* st = new detail::handle_state();
+  try {
+    check_hip(hipGetDevice(&st->device), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&st->num_cus, hipDeviceAttributeMultiprocessorCount, st->device),
+              "hipDeviceGetAttribute");
+    st->table = kernel_table();
+    for (auto& a : st->table.shgemm)
+      for (auto& b : a)
+        for (auto& c : b)
+          for (auto& d : c)
+            for (auto& e : d)
+              for (auto& f : e)
+                for (auto& g : f)
+                  if (g.fn) set_lds_limit(g);
+    check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
+    st->ws_bytes = kWorkspaceBytes;
+  } catch (...) {
+    if (st->ws) (void)hipFree(st->ws);
+    delete st;
+    throw;
+  }
+  const char* sc = std::getenv("MTK_SHGEMM_EXP_SCALING");
+  st->exp_scaling = (sc != nullptr && sc[0] == '1');
+  handle.state = st;
+}
+
+void destroy(shgemmHandle_t& handle) {
+  if (handle.state == nullptr) return;
+  auto* st = handle.state;
+  // Work enqueued by this handle may still read the scratch buffers.
+  (void)hipDeviceSynchronize();
+  if (st->ws) (void)hipFree(st->ws);
+  if (st->row_max) (void)hipFree(st->row_max);
+  if (st->row_exp) (void)hipFree(st->row_exp);
+  delete st;
+  handle.state = nullptr;
+}
+
+void enable_kernel_level_fixing(shgemmHandle_t& handle, const detail::kernel_level kernel_level) {
+  if (kernel_level != detail::P0 && kernel_level != detail::P1) throw error("kernel level must be P0 or P1");
+  handle.fixed_lernel_level = kernel_level;
+}
+
+void disable_kernel_level_fixing(shgemmHandle_t& handle) { handle.fixed_lernel_level = detail::num_levels; }
+
+void set_debug_mode(shgemmHandle_t& handle, const unsigned on) { handle.debug_mode = on; }
+
+void set_cuda_stream(shgemmHandle_t& handle, hipStream_t const cuda_stream) { handle.cuda_stream = cuda_stream; }
+
+void set_exponent_scaling(shgemmHandle_t& handle, const bool on) {
+  if (handle.state == nullptr) throw error("handle not created");
+  handle.state->exp_scaling = on;
+}
+
+detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
+                            const std::size_t m, const std::size_t n, const std::size_t k, const float* const alpha_ptr,
+                            const float* const a_ptr, const std::size_t lda, const half* const b_ptr,
+                            const std::size_t ldb, const float* const beta_ptr, float* const c_ptr,
+                            const std::size_t ldc, const tc_t compute_type, const operation_t op_c) {
+  validate_common(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
+  if (compute_type != fp16 && compute_type != tf32) throw error("invalid tc_t");
+  detail::handle_state* st = handle.state;
+  const hipStream_t stream = handle.cuda_stream;
+  if (m == 0 || n == 0) return detail::P0;
+
+  const float alpha = *alpha_ptr, beta = *beta_ptr;
+  const int inner = (int)(op_c == op_n ? m : n);
+  const int64_t elems = (int64_t)m * (int64_t)n;
+  if (k == 0 || alpha == 0.0f) {
+    // C = beta * C without touching A or B (BLAS semantics)
+    check_hip(shgemm_amd::launch_scale_c(elems, inner, beta, c_ptr, (int64_t)ldc, stream), "scale kernel");
+    return detail::P0;
+  }
+
+  const int mode = compute_type == fp16 ? shgemm_amd::MODE_FP16 : shgemm_amd::MODE_BF16;
+  const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int scaled = (mode == shgemm_amd::MODE_FP16 && st->exp_scaling) ? 1 : 0;
+
+  shgemm_amd::GemmParams p{};
+  p.m = (int)m;
+  p.n = (int)n;
+  p.k = (int)k;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.a = a_ptr;
+  p.lda = (int64_t)lda;
+  p.b = b_ptr;
+  p.ldb = (int64_t)ldb;
+  p.c = c_ptr;
+  p.ldc = (int64_t)ldc;
+  // Vector loads: 16 bytes of FP32 for A; 16 (k-contiguous) or 8 bytes of FP16 for B.
+  p.vec_a = aligned(a_ptr, 16) && (lda % 4 == 0);
+  p.vec_b = (ob == shgemm_amd::OP_N) ? (aligned(b_ptr, 16) && ldb % 8 == 0) : (aligned(b_ptr, 8) && ldb % 4 == 0);
+
+  if (scaled) {
+    if (st->row_cap < m) {
+      // Grow the prescale scratch; earlier work on this handle may still use it.
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (st->row_max) (void)hipFree(st->row_max);
+      if (st->row_exp) (void)hipFree(st->row_exp);
+      st->row_max = nullptr;
+      st->row_exp = nullptr;
+      st->row_cap = 0;
+      check_hip(hipMalloc(&st->row_max, m * sizeof(uint32_t)), "hipMalloc(row scratch)");
+      check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
+      st->row_cap = m;
+    }
+    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp, stream),
+              "row exponent pass");
+    p.row_exp = st->row_exp;
+  }
+
+  const auto& tab = st->table.shgemm[oa][ob][oc][mode];
+  const int ncu = std::max(1, st->num_cus);
+
+  // Split-K when the output is too small to fill the GPU and k is long
+  // (reference shgemm.cu:532 uses m*n <= 884,736 && k >= 4096 && beta == 0;
+  // here beta is applied by the reduction, so beta != 0 is allowed too).
+  {
+    const auto& e = tab[shgemm_amd::LEVEL_P1][1][scaled];
+    const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
+    if (tiles < ncu && k >= 4096) {
+      // Slices: enough workgroups for two per CU, each slice at least 8 k-tiles,
+      // all partials inside the workspace.  Depends on the shape and CU count only.
+      int64_t slices = 1;
+      while (tiles * slices < 2 * (int64_t)ncu && (int64_t)k / (slices * 2) >= 8 * e.bk &&
+             (slices * 2) * elems * (int64_t)sizeof(float) <= (int64_t)st->ws_bytes)
+        slices *= 2;
+      if (slices > 1) {
+        const int64_t kslice = (((int64_t)k + slices - 1) / slices + e.bk - 1) / e.bk * e.bk;
+        const int64_t used = ((int64_t)k + kslice - 1) / kslice;
+        p.k_slice = (int)kslice;
+        p.ws = st->ws;
+        p.tiles_m = (int)((m + e.bm - 1) / e.bm);
+        p.tiles_n = (int)((n + e.bn - 1) / e.bn);
+        p.group_m = group_rows(p.tiles_m);
+        if (handle.debug_mode)
+          std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] split-K tile=%dx%d slices=%lld\n", m, n,
+                      k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn,
+                      (long long)used);
+        launch(e, dim3((unsigned)tiles, (unsigned)used), p, stream);
+        check_hip(shgemm_amd::launch_splitk_reduce(st->ws, (int)used, elems, inner, alpha, beta, c_ptr,
+                                                   (int64_t)ldc, stream),
+                  "split-K reduction");
+        return detail::P0;
+      }
+    }
+  }
+
+  // Tile level (reference shgemm.cu:643-656): the large tiles when they give
+  // at least two waves of workgroups, else the small ones.
+  int level = detail::P1;
+  if (handle.fixed_lernel_level < detail::num_levels) {
+    level = handle.fixed_lernel_level;
+  } else {
+    const auto& e = tab[shgemm_amd::LEVEL_P1][0][scaled];
+    const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
+    const int64_t filling = 2 * (int64_t)ncu;  // two workgroups per CU
+    level = tiles >= 2 * filling ? detail::P1 : detail::P0;
+  }
+  const auto& e = tab[level][0][scaled];
+  p.tiles_m = (int)((m + e.bm - 1) / e.bm);
+  p.tiles_n = (int)((n + e.bn - 1) / e.bn);
+  p.group_m = group_rows(p.tiles_m);
+  const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+  if (tiles > INT_MAX) throw error("too many output tiles");
+  if (handle.debug_mode)
+    std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d kernel_level=%d%s\n", m, n, k,
+                op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk, level,
+                scaled ? " prescaled" : "");
+  launch(e, dim3((unsigned)tiles), p, stream);
+  return static_cast<detail::kernel_level>(level);
+}
+
+void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b, const std::size_t m,
+                const std::size_t n, const std::size_t k, const float* const alpha_ptr, const float* const a_ptr,
+                const std::size_t lda, const float* const b_ptr, const std::size_t ldb, const float* const beta_ptr,
+                float* const c_ptr, const std::size_t ldc, const operation_t op_c) {
+  validate_common(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
+  detail::handle_state* st = handle.state;
+  const hipStream_t stream = handle.cuda_stream;
+  if (m == 0 || n == 0) return;
+  const float alpha = *alpha_ptr, beta = *beta_ptr;
+  const int inner = (int)(op_c == op_n ? m : n);
+  if (k == 0 || alpha == 0.0f) {
+    check_hip(shgemm_amd::launch_scale_c((int64_t)m * (int64_t)n, inner, beta, c_ptr, (int64_t)ldc, stream),
+              "scale kernel");
+    return;
+  }
+  const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const auto& e = st->table.sgemm[oa][ob][oc];
+  shgemm_amd::GemmParams p{};
+  p.m = (int)m;
+  p.n = (int)n;
+  p.k = (int)k;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.a = a_ptr;
+  p.lda = (int64_t)lda;
+  p.b = b_ptr;
+  p.ldb = (int64_t)ldb;
+  p.c = c_ptr;
+  p.ldc = (int64_t)ldc;
+  p.vec_a = aligned(a_ptr, 16) && lda % 4 == 0;
+  p.vec_b = aligned(b_ptr, 16) && ldb % 4 == 0;
+  p.tiles_m = (int)((m + e.bm - 1) / e.bm);
+  p.tiles_n = (int)((n + e.bn - 1) / e.bn);
+  p.group_m = group_rows(p.tiles_m);
+  const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+  if (tiles > INT_MAX) throw error("too many output tiles");
+  if (handle.debug_mode)
+    std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s] sgemm_fp32 tile=%dx%dx%d\n", m, n, k, op_name(op_a),
+                op_name(op_b), e.bm, e.bn, e.bk);
+  launch(e, dim3((unsigned)tiles), p, stream);
+}
+
+}  // namespace shgemm
+}  // namespace mtk
+
+namespace shgemm_amd {
+void register_all(KernelTable& t) {
+  register_shgemm_fp16_nn(t);
+  register_shgemm_fp16_nt(t);
+  register_shgemm_fp16_tn(t);
+  register_shgemm_fp16_tt(t);
+  register_shgemm_bf16_nn(t);
+  register_shgemm_bf16_nt(t);
+  register_shgemm_bf16_tn(t);
+  register_shgemm_bf16_tt(t);
+  register_sgemm(t);
+}
+}  // namespace shgemm_amd

@@ -1,0 +1,548 @@
+// Error-corrected SHGEMM kernel for gfx950 (CDNA4), plus the deterministic
+// split-K reduction and the per-row exponent pass.
+//
+// Reference counterpart: src/shgemm.cu:36-119 (kernel entry),
+// src/shgemm_pipeline_core.hpp:108-184 (k-loop), src/shgemm_core.hpp:48-75
+// (per-warp MMA loop), src/wmmae_shgemm.hpp:11-125 (split + two products),
+// src/dmem_accessor.hpp:187-356 (alpha/beta epilogue).
+//
+// Structure (one workgroup = 4 wavefronts of 64 lanes, one BM x BN tile of C):
+//   * global -> registers: the FP32 operand and the FP16 operand of the next
+//     k-tile are loaded while the current one is multiplied;
+//   * registers -> LDS: the FP32 operand is split here, once per workgroup,
+//     into FP16 hi and 2^11-scaled FP16 lo (or three BF16 parts); both are
+//     written k-chunk-major (see lds_slot) so each MFMA operand is one
+//     16-byte LDS read per lane whatever the global layout was;
+//   * LDS -> MFMA: v_mfma_f32_32x32x16_{f16,bf16}, FP32 accumulation, the hi
+//     and lo products in separate accumulators;
+//   * epilogue: C = alpha * (hi + 2^-11 lo) + beta * C, or the slice partial
+//     into the split-K workspace.
+#pragma once
+#include "gemm_common.hpp"
+
+namespace shgemm_amd {
+
+// ---------------------------------------------------------------------------
+// FP32 operand tile: R rows x BK k-values, 256 threads.
+// Element (r, k) sits at base[r*ld + k] when KCONT (k contiguous) and at
+// base[r + k*ld] otherwise.
+//   KCONT item = (row r, chunk of 8 k): two 16-byte loads.
+//   !KCONT item = (4 rows, 4 k): four 16-byte loads down 4 columns.
+// ---------------------------------------------------------------------------
+template <bool KCONT, int R, int BK>
+struct TileF32 {
+  static constexpr int KC = BK / 8;
+  static constexpr int NITEMS = KCONT ? R * KC : (R / 4) * KC * 2;
+  static constexpr int PER = NITEMS / 256;
+  static_assert(NITEMS % 256 == 0 && PER >= 1, "tile must split evenly over 256 threads");
+  static constexpr int NV = KCONT ? 8 : 16;
+  float v[PER][NV];
+  float scale[PER][KCONT ? 1 : 4];  // per-row prescale 2^-e(row)
+
+  __device__ __forceinline__ static void item(int i, int& r, int& k) {
+    if constexpr (KCONT) {
+      r = i / KC;
+      k = 8 * (i % KC);
+    } else {
+      r = 4 * (i % (R / 4));
+      k = 4 * (i / (R / 4));
+    }
+  }
+
+  __device__ __forceinline__ void init_scale(const int* row_exp, int r0, int rlim, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+#pragma unroll
+      for (int e = 0; e < (KCONT ? 1 : 4); ++e) {
+        const int rr = r0 + r + e;
+        scale[p][e] = (row_exp != nullptr && rr < rlim) ? __builtin_ldexpf(1.0f, -row_exp[rr]) : 1.0f;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void load_fast(const float* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      if constexpr (KCONT) {
+        const float* ptr = base + (int64_t)(r0 + r) * ld + (k0 + k);
+        const f32x4 x = *reinterpret_cast<const f32x4*>(ptr);
+        const f32x4 y = *reinterpret_cast<const f32x4*>(ptr + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[p][e] = x[e];
+          v[p][4 + e] = y[e];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const f32x4 x = *reinterpret_cast<const f32x4*>(base + (r0 + r) + (int64_t)(k0 + k + j) * ld);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[p][4 * j + e] = x[e];
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void load_safe(const float* __restrict__ base, int64_t ld, int r0, int k0,
+                                            int rlim, int klim, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      if constexpr (KCONT) {
+        const int rr = r0 + r;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int kk = k0 + k + e;
+          v[p][e] = (rr < rlim && kk < klim) ? base[(int64_t)rr * ld + kk] : 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int rr = r0 + r + e, kk = k0 + k + j;
+            v[p][4 * j + e] = (rr < rlim && kk < klim) ? base[rr + (int64_t)kk * ld] : 0.0f;
+          }
+        }
+      }
+    }
+  }
+
+  // FP16 hi / lo split into two LDS arrays of R*KC chunks.
+  template <bool SCALED>
+  __device__ __forceinline__ void store_f16(uint4* __restrict__ hi, uint4* __restrict__ lo, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      const int c = k >> 3;
+      if constexpr (KCONT) {
+        f16x8 h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          _Float16 hh, ll;
+          split_f16(SCALED ? v[p][e] * scale[p][0] : v[p][e], hh, ll);
+          h[e] = hh;
+          l[e] = ll;
+        }
+        const int s = lds_slot<R>(c, r);
+        *reinterpret_cast<f16x8*>(&hi[s]) = h;
+        *reinterpret_cast<f16x8*>(&lo[s]) = l;
+      } else {
+        const int kh = (k >> 2) & 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          f16x4 h, l;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            _Float16 hh, ll;
+            split_f16(SCALED ? v[p][4 * j + e] * scale[p][e] : v[p][4 * j + e], hh, ll);
+            h[j] = hh;
+            l[j] = ll;
+          }
+          const int s = lds_slot<R>(c, r + e);
+          reinterpret_cast<f16x4*>(&hi[s])[kh] = h;
+          reinterpret_cast<f16x4*>(&lo[s])[kh] = l;
+        }
+      }
+    }
+  }
+
+  // Three-part BF16 split into three LDS arrays.
+  __device__ __forceinline__ void store_bf16(uint4* __restrict__ d0, uint4* __restrict__ d1,
+                                             uint4* __restrict__ d2, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      const int c = k >> 3;
+      if constexpr (KCONT) {
+        bf16x8 x0, x1, x2;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          __bf16 a0, a1, a2;
+          split_bf16x3(v[p][e], a0, a1, a2);
+          x0[e] = a0;
+          x1[e] = a1;
+          x2[e] = a2;
+        }
+        const int s = lds_slot<R>(c, r);
+        *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
+        *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
+        *reinterpret_cast<bf16x8*>(&d2[s]) = x2;
+      } else {
+        const int kh = (k >> 2) & 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          bf16x4 x0, x1, x2;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            __bf16 a0, a1, a2;
+            split_bf16x3(v[p][4 * j + e], a0, a1, a2);
+            x0[j] = a0;
+            x1[j] = a1;
+            x2[j] = a2;
+          }
+          const int s = lds_slot<R>(c, r + e);
+          reinterpret_cast<bf16x4*>(&d0[s])[kh] = x0;
+          reinterpret_cast<bf16x4*>(&d1[s])[kh] = x1;
+          reinterpret_cast<bf16x4*>(&d2[s])[kh] = x2;
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FP16 operand tile (raw binary16 bits), same item scheme as TileF32.
+//   KCONT item = (row, chunk of 8 k): one 16-byte load.
+//   !KCONT item = (4 rows, 4 k): four 8-byte loads, transposed in registers.
+// ---------------------------------------------------------------------------
+template <bool KCONT, int R, int BK>
+struct TileF16 {
+  static constexpr int KC = BK / 8;
+  static constexpr int NITEMS = KCONT ? R * KC : (R / 4) * KC * 2;
+  static constexpr int PER = NITEMS / 256;
+  static_assert(NITEMS % 256 == 0 && PER >= 1, "tile must split evenly over 256 threads");
+  // KCONT: v[p][0] holds 8 halves; !KCONT: v[p][j] (j<4) holds rows r..r+3 at k+j
+  u32x4 vk[KCONT ? PER : 1];
+  u32x2 vr[KCONT ? 1 : PER][4];
+
+  __device__ __forceinline__ static void item(int i, int& r, int& k) {
+    if constexpr (KCONT) {
+      r = i / KC;
+      k = 8 * (i % KC);
+    } else {
+      r = 4 * (i % (R / 4));
+      k = 4 * (i / (R / 4));
+    }
+  }
+
+  __device__ __forceinline__ void load_fast(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      if constexpr (KCONT) {
+        vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)(r0 + r) * ld + (k0 + k));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          vr[p][j] = *reinterpret_cast<const u32x2*>(base + (r0 + r) + (int64_t)(k0 + k + j) * ld);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void load_safe(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
+                                            int rlim, int klim, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      if constexpr (KCONT) {
+        const int rr = r0 + r;
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int kk = k0 + k + 2 * q;
+          const uint32_t x0 = (rr < rlim && kk < klim) ? base[(int64_t)rr * ld + kk] : 0u;
+          const uint32_t x1 = (rr < rlim && kk + 1 < klim) ? base[(int64_t)rr * ld + kk + 1] : 0u;
+          w[q] = x0 | (x1 << 16);
+        }
+        vk[p] = u32x4{w[0], w[1], w[2], w[3]};
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int kk = k0 + k + j;
+          uint32_t x[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int rr = r0 + r + e;
+            x[e] = (rr < rlim && kk < klim) ? base[rr + (int64_t)kk * ld] : 0u;
+          }
+          vr[p][j] = u32x2{x[0] | (x[1] << 16), x[2] | (x[3] << 16)};
+        }
+      }
+    }
+  }
+
+  // half e (0..3) of a 4-half word pair
+  __device__ __forceinline__ static uint32_t half_of(const u32x2& w, int e) {
+    return ((e < 2 ? w[0] : w[1]) >> (16 * (e & 1))) & 0xffffu;
+  }
+
+  __device__ __forceinline__ void store_raw(uint4* __restrict__ dst, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      const int c = k >> 3;
+      if constexpr (KCONT) {
+        *reinterpret_cast<u32x4*>(&dst[lds_slot<R>(c, r)]) = vk[p];
+      } else {
+        const int kh = (k >> 2) & 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u32x2 o{half_of(vr[p][0], e) | (half_of(vr[p][1], e) << 16),
+                        half_of(vr[p][2], e) | (half_of(vr[p][3], e) << 16)};
+          reinterpret_cast<u32x2*>(&dst[lds_slot<R>(c, r + e)])[kh] = o;
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store_bf16(uint4* __restrict__ d0, uint4* __restrict__ d1, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, k;
+      item(tid + 256 * p, r, k);
+      const int c = k >> 3;
+      if constexpr (KCONT) {
+        bf16x8 x0, x1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t h = (vk[p][e >> 1] >> (16 * (e & 1))) & 0xffffu;
+          __bf16 b0, b1;
+          split_bf16x2(bits_to_f16((uint16_t)h), b0, b1);
+          x0[e] = b0;
+          x1[e] = b1;
+        }
+        const int s = lds_slot<R>(c, r);
+        *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
+        *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
+      } else {
+        const int kh = (k >> 2) & 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          bf16x4 x0, x1;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            __bf16 b0, b1;
+            split_bf16x2(bits_to_f16((uint16_t)half_of(vr[p][j], e)), b0, b1);
+            x0[j] = b0;
+            x1[j] = b1;
+          }
+          const int s = lds_slot<R>(c, r + e);
+          reinterpret_cast<bf16x4*>(&d0[s])[kh] = x0;
+          reinterpret_cast<bf16x4*>(&d1[s])[kh] = x1;
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Tile configuration
+// ---------------------------------------------------------------------------
+template <int BM_, int BN_, int BK_, int WM_, int WN_>
+struct TileCfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+  static constexpr int KC = BK / 8;
+  static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
+  static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 16 == 0, "MFMA 32x32x16 granularity");
+  template <int MODE>
+  static constexpr int stage_chunks() {
+    return (MODE == MODE_FP16 ? 2 : 3) * BM * KC + (MODE == MODE_FP16 ? 1 : 2) * BN * KC;
+  }
+  template <int MODE>
+  static constexpr size_t lds_bytes() {
+    return 2 * (size_t)stage_chunks<MODE>() * 16;
+  }
+};
+
+// P1: large tiles; P0: small tiles (the reference's two kernel levels,
+// include/shgemm/shgemm.hpp:30-34).
+using CfgP1 = TileCfg<128, 128, 32, 64, 64>;
+using CfgP0 = TileCfg<64, 64, 64, 32, 32>;
+
+// ---------------------------------------------------------------------------
+// The kernel.
+//   OPA, OPB: operand storage (OP_N / OP_T), OPC: C storage.
+//   SCALED: A rows are prescaled by 2^-row_exp (fp16 mode only).
+//   SPLITK: blockIdx.y selects a k-slice; the partial goes to p.ws.
+// ---------------------------------------------------------------------------
+template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK, bool SCALED>
+__global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
+  constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM;
+  constexpr int NA = MODE == MODE_FP16 ? 2 : 3;
+  constexpr int A_CH = BM * KC, B_CH = BN * KC;
+  constexpr int STAGE = CFG::template stage_chunks<MODE>();
+  // With C column-major the MFMA's lane index runs along m (coalesced C
+  // stores); with C transposed it runs along n.
+  constexpr bool LANE_M = (OPC == OP_N);
+  extern __shared__ uint4 lds[];
+
+  TileF32<OPA == OP_T, BM, BK> ta;
+  TileF16<OPB == OP_N, BN, BK> tb;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int lrow = lane & 31, lh = lane >> 5;
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  int kbeg = 0, kend = p.k;
+  if constexpr (SPLITK) {
+    kbeg = blockIdx.y * p.k_slice;
+    kend = min(p.k, kbeg + p.k_slice);
+  }
+  const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  const float* __restrict__ A = p.a;
+  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+  const bool a_full = p.vec_a && (m0 + BM <= p.m);
+  const bool b_full = p.vec_b && (n0 + BN <= p.n);
+
+  if constexpr (SCALED) ta.init_scale(p.row_exp, m0, p.m, tid);
+
+  f32x16 acc[2][FM][FN];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[h][i][j][e] = 0.0f;
+
+  auto gload = [&](int kt) {
+    const int k0 = kbeg + kt * BK;
+    const bool kf = (k0 + BK <= kend);
+    if (a_full && kf) ta.load_fast(A, p.lda, m0, k0, tid);
+    else ta.load_safe(A, p.lda, m0, k0, p.m, kend, tid);
+    if (b_full && kf) tb.load_fast(B, p.ldb, n0, k0, tid);
+    else tb.load_safe(B, p.ldb, n0, k0, p.n, kend, tid);
+  };
+  auto sstore = [&](int stage) {
+    uint4* st = lds + stage * STAGE;
+    if constexpr (MODE == MODE_FP16) {
+      ta.template store_f16<SCALED>(st, st + A_CH, tid);
+      tb.store_raw(st + NA * A_CH, tid);
+    } else {
+      ta.store_bf16(st, st + A_CH, st + 2 * A_CH, tid);
+      tb.store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
+    }
+  };
+  auto compute = [&](const uint4* __restrict__ st) {
+#pragma unroll
+    for (int s = 0; s < BK / 16; ++s) {
+      const int c = 2 * s + lh;
+      if constexpr (MODE == MODE_FP16) {
+        f16x8 ah[FM], al[FM], bb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int sl = lds_slot<BM>(c, wm * WM + i * 32 + lrow);
+          ah[i] = *reinterpret_cast<const f16x8*>(&st[sl]);
+          al[i] = *reinterpret_cast<const f16x8*>(&st[A_CH + sl]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          bb[j] = *reinterpret_cast<const f16x8*>(&st[NA * A_CH + lds_slot<BN>(c, wn * WN + j * 32 + lrow)]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            if constexpr (LANE_M) {
+              acc[0][i][j] = mfma_f16(bb[j], ah[i], acc[0][i][j]);
+              acc[1][i][j] = mfma_f16(bb[j], al[i], acc[1][i][j]);
+            } else {
+              acc[0][i][j] = mfma_f16(ah[i], bb[j], acc[0][i][j]);
+              acc[1][i][j] = mfma_f16(al[i], bb[j], acc[1][i][j]);
+            }
+          }
+      } else {
+        bf16x8 a0[FM], a1[FM], a2[FM], b0[FN], b1[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int sl = lds_slot<BM>(c, wm * WM + i * 32 + lrow);
+          a0[i] = *reinterpret_cast<const bf16x8*>(&st[sl]);
+          a1[i] = *reinterpret_cast<const bf16x8*>(&st[A_CH + sl]);
+          a2[i] = *reinterpret_cast<const bf16x8*>(&st[2 * A_CH + sl]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int sl = lds_slot<BN>(c, wn * WN + j * 32 + lrow);
+          b0[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + sl]);
+          b1[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + B_CH + sl]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            // acc0: a0*b0; acc1: a0*b1 + a1*b0 + a1*b1 + a2*b0 (a2*b1 ~ 2^-24 dropped)
+            if constexpr (LANE_M) {
+              acc[0][i][j] = mfma_bf16(b0[j], a0[i], acc[0][i][j]);
+              acc[1][i][j] = mfma_bf16(b1[j], a0[i], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(b0[j], a1[i], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(b1[j], a1[i], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(b0[j], a2[i], acc[1][i][j]);
+            } else {
+              acc[0][i][j] = mfma_bf16(a0[i], b0[j], acc[0][i][j]);
+              acc[1][i][j] = mfma_bf16(a0[i], b1[j], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(a1[i], b0[j], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(a1[i], b1[j], acc[1][i][j]);
+              acc[1][i][j] = mfma_bf16(a2[i], b0[j], acc[1][i][j]);
+            }
+          }
+      }
+    }
+  };
+
+  // Register-staged double buffer: one barrier per k-tile.  The store into
+  // stage cur^1 is safe because the barrier closing the previous iteration
+  // ended every read of that stage.
+  if (ktiles > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < ktiles) gload(kt + 1);
+    compute(lds + cur * STAGE);
+    if (kt + 1 < ktiles) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue.  Lane/register -> (m, n) of the 32x32 MFMA output block:
+  // lane index gives the column, register j gives row (j/4)*8 + (lane/32)*4 + j%4.
+  constexpr float lo_w = MODE == MODE_FP16 ? (1.0f / 2048.0f) : 1.0f;
+  const float alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int ridx = (e >> 2) * 8 + lh * 4 + (e & 3);
+        int mm, nn;
+        if constexpr (LANE_M) {
+          mm = m0 + wm * WM + i * 32 + lrow;
+          nn = n0 + wn * WN + j * 32 + ridx;
+        } else {
+          mm = m0 + wm * WM + i * 32 + ridx;
+          nn = n0 + wn * WN + j * 32 + lrow;
+        }
+        if (mm < p.m && nn < p.n) {
+          float v = __builtin_fmaf(acc[1][i][j][e], lo_w, acc[0][i][j][e]);
+          if constexpr (SCALED) v = __builtin_ldexpf(v, p.row_exp[mm]);
+          if constexpr (SPLITK) {
+            const int64_t off = (int64_t)blockIdx.y * p.m * p.n +
+                                (LANE_M ? (mm + (int64_t)nn * p.m) : (nn + (int64_t)mm * p.n));
+            p.ws[off] = v;
+          } else {
+            float* cp = p.c + (LANE_M ? (mm + nn * p.ldc) : (nn + mm * p.ldc));
+            *cp = (beta == 0.0f) ? alpha * v : alpha * v + beta * *cp;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace shgemm_amd

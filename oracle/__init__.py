@@ -53,6 +53,9 @@ def lib() -> ctypes.CDLL:
                                            i64, i64, i32]
         L.oracle_split_f16.argtypes = [vp, vp, vp, i64]
         L.oracle_split_bf16x3.argtypes = [vp, vp, vp, vp, i64]
+        L.oracle_split_f16_same_scale.argtypes = [vp, vp, vp, i64]
+        L.oracle_row_exponent.argtypes = [vp, i64, i64]
+        L.oracle_row_exponent.restype = i32
         L.oracle_split_bf16x2.argtypes = [vp, vp, vp, i64]
         L.oracle_f32_to_f16.argtypes = [f32]
         L.oracle_f32_to_f16.restype = ctypes.c_uint16
@@ -128,6 +131,32 @@ def split_f16(a: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return hi, lo
 
 
+def row_exponents(Am: np.ndarray) -> np.ndarray:
+    """Per-row exponent e(row) of the prescaled fp16 split for an m x k matrix."""
+    Am = np.ascontiguousarray(Am, dtype=np.float32)
+    out = np.empty(Am.shape[0], np.int64)
+    for r in range(Am.shape[0]):
+        out[r] = lib().oracle_row_exponent(Am[r].ctypes.data, Am.shape[1], 1)
+    return out
+
+
+def split_f16_same_scale(x: np.ndarray):
+    x = _check_f32(x.ravel(), "x")
+    hi = np.empty(x.size, np.uint16)
+    lo = np.empty(x.size, np.uint16)
+    lib().oracle_split_f16_same_scale(_ptr(x), _ptr(hi), _ptr(lo), x.size)
+    return hi, lo
+
+
+def split_value_f16_rowscaled(Am: np.ndarray) -> np.ndarray:
+    """FP64 value of the prescaled split (hi + lo) * 2^e(row) of an m x k matrix."""
+    e = row_exponents(Am)
+    xs = np.ldexp(Am.astype(np.float32), -e[:, None]).astype(np.float32)
+    hi, lo = split_f16_same_scale(xs)
+    v = f16_bits_to_f32(hi).astype(np.float64) + f16_bits_to_f32(lo).astype(np.float64)
+    return np.ldexp(v.reshape(Am.shape), e[:, None])
+
+
 def split_bf16x3(a: np.ndarray):
     a = _check_f32(a.ravel(), "a")
     p = [np.empty(a.size, np.uint16) for _ in range(3)]
@@ -159,13 +188,24 @@ def split_value_f16(a: np.ndarray) -> np.ndarray:
 def emulate_shgemm(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, compute_type=0):
     """FP64 product of the operands exactly as the GPU kernel represents them.
 
-    compute_type 0 (fp16): A -> hi + 2^-11 lo (FP16 each), B as given.
+    compute_type 0 (fp16, prescaling off): A -> hi + 2^-11 lo (FP16 each).
     compute_type 1 (tf32): A -> a0 + a1 + a2, B -> b0 + b1 (BF16 each); the
     kernel forms a0b0 + a0b1 + a1b0 + a1b1 + a2b0 (a2b1 is dropped).
+    compute_type 2 (fp16, prescaled): row r of op(A) -> (hi + lo) * 2^e(r).
     The GPU result differs from this only by FP32 accumulation rounding.
     """
     A = _check_f32(A, "A")
     B = _check_f32(B, "B")
+    if compute_type == 2:
+        Am = np.array(op_view(A, op_a, m, k, lda))
+        Ae = split_value_f16_rowscaled(Am)
+        Bm = op_view(B.astype(np.float64), op_b, k, n, ldb)
+        out = np.zeros((ldc * n) if op_c == OP_N else (ldc * m), dtype=np.float64)
+        view = _as_c_view(out, op_c, m, n, ldc)
+        view[...] = alpha * (Ae @ Bm)
+        if beta != 0.0 and C0 is not None:
+            view += beta * _as_c_view(C0.astype(np.float64), op_c, m, n, ldc)
+        return out
     if compute_type == 0:
         Ae = split_value_f16(A).astype(np.float64)
         Be = B.astype(np.float64)

@@ -13,6 +13,9 @@
  *   oracle_split_f16 hi = fp16(a), lo = fp16((a - hi) * 2^11), both
  *                    round-to-nearest-even (reference src/wmmae_shgemm.hpp:29-31;
  *                    method of Ootomo & Yokota, IJHPCA 2022).
+ *   oracle_split_f16_same_scale / oracle_row_exponent  the prescaled fp16
+ *                    split: each row of A scaled by 2^-e(row), e = ilogb(max|row|)
+ *                    - 14, then hi = fp16(x), lo = fp16(x - hi).
  *   oracle_split_bf16x3 / oracle_split_bf16x2  the BF16 parts of the tf32
  *                    compute type.
  *
@@ -106,6 +109,29 @@ void oracle_split_f16(const float* a, uint16_t* hi, uint16_t* lo, int64_t n) {
     hi[i] = h;
     lo[i] = oracle_f32_to_f16(r * 2048.0f);
   }
+}
+
+/* Prescaled variant (row already multiplied by 2^-e): hi = fp16(x),
+ * lo = fp16(x - hi), both at the same scale. */
+void oracle_split_f16_same_scale(const float* x, uint16_t* hi, uint16_t* lo, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const uint16_t h = oracle_f32_to_f16(x[i]);
+    hi[i] = h;
+    lo[i] = oracle_f32_to_f16(x[i] - oracle_f16_to_f32(h));
+  }
+}
+
+/* Row exponent of the prescaled fp16 split: ilogb(max |row|) - 14, clamped
+ * to >= -126; 0 for an all-zero or non-finite row. */
+int oracle_row_exponent(const float* row, int64_t n, int64_t stride) {
+  float mx = 0.0f;
+  for (int64_t i = 0; i < n; ++i) {
+    const float v = fabsf(row[i * stride]);
+    if (!(v <= mx)) mx = v; /* also picks up NaN */
+  }
+  if (mx == 0.0f || !isfinite(mx)) return 0;
+  const int e = ilogbf(mx) - 14;
+  return e < -126 ? -126 : e;
 }
 
 void oracle_split_bf16x3(const float* a, uint16_t* p0, uint16_t* p1, uint16_t* p2, int64_t n) {

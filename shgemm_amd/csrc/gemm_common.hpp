@@ -15,9 +15,16 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 enum : int { OP_N = 0, OP_T = 1 };
-// MODE_FP16: A = hi + 2^-11 lo, both FP16 (reference src/wmmae_shgemm.hpp:29-31)
-// MODE_BF16: A = a0 + a1 + a2 and B = b0 + b1, all BF16 (the tf32 compute type)
-enum : int { MODE_FP16 = 0, MODE_BF16 = 1 };
+// MODE_FP16:  A = hi + 2^-11 lo, both FP16, hi and lo products in separate
+//             FP32 accumulators (reference src/wmmae_shgemm.hpp:29-31, 73-125).
+// MODE_BF16:  A = a0 + a1 + a2 and B = b0 + b1, all BF16 (the tf32 compute
+//             type); the five products share one accumulator.
+// MODE_FP16S: each row of A prescaled by 2^-e(row) (exact), then
+//             x = hi + lo with both FP16 at the same scale, so the hi and lo
+//             products share one FP32 accumulator; C's row is scaled back by
+//             2^e(row).  Same split precision as MODE_FP16, FP32 exponent range.
+enum : int { MODE_FP16 = 0, MODE_BF16 = 1, MODE_FP16S = 2 };
+constexpr int kNumModes = 3;
 
 // Everything a GEMM launch needs.  Sizes are 32-bit (the host rejects larger
 // ones); leading dimensions and offsets are 64-bit.
@@ -73,6 +80,12 @@ __device__ __forceinline__ void split_f16(float a, _Float16& hi, _Float16& lo) {
   hi = (_Float16)a;
   const float r = a - (float)hi;
   lo = (_Float16)(r * 2048.0f);
+}
+
+// Prescaled FP32 -> FP16 hi + FP16 residual at the same scale.
+__device__ __forceinline__ void split_f16_same_scale(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
 }
 
 // FP32 -> three BF16 parts a0 + a1 + a2 (each residual exact in FP32).

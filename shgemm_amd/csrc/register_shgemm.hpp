@@ -7,26 +7,23 @@
 
 namespace shgemm_amd {
 
-template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK, bool SCALED>
+template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
 inline void put_shgemm(KernelTable& t, int level) {
-  KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][level][SPLITK ? 1 : 0][SCALED ? 1 : 0];
-  e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OPA, OPB, OPC, MODE, CFG, SPLITK, SCALED>);
+  KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][level][SPLITK ? 1 : 0];
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OPA, OPB, OPC, MODE, CFG, SPLITK>);
   e.lds = CFG::template lds_bytes<MODE>();
   e.bm = CFG::BM;
   e.bn = CFG::BN;
   e.bk = CFG::BK;
+  e.mode = MODE;
 }
 
 template <int OPA, int OPB, int OPC, int MODE>
 inline void put_shgemm_opc(KernelTable& t) {
-  put_shgemm<OPA, OPB, OPC, MODE, CfgP1, false, false>(t, LEVEL_P1);
-  put_shgemm<OPA, OPB, OPC, MODE, CfgP0, false, false>(t, LEVEL_P0);
-  put_shgemm<OPA, OPB, OPC, MODE, CfgP1, true, false>(t, LEVEL_P1);
-  if constexpr (MODE == MODE_FP16) {
-    put_shgemm<OPA, OPB, OPC, MODE, CfgP1, false, true>(t, LEVEL_P1);
-    put_shgemm<OPA, OPB, OPC, MODE, CfgP0, false, true>(t, LEVEL_P0);
-    put_shgemm<OPA, OPB, OPC, MODE, CfgP1, true, true>(t, LEVEL_P1);
-  }
+  using C = ModeCfg<MODE>;
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, false>(t, LEVEL_P1);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, false>(t, LEVEL_P0);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::SK, true>(t, LEVEL_P1);
 }
 
 template <int OPA, int OPB, int MODE>

@@ -11,15 +11,21 @@ struct KernelEntry {
   const void* fn = nullptr;
   size_t lds = 0;      // dynamic LDS bytes
   int bm = 0, bn = 0, bk = 0;
+  int mode = -1;
 };
 
 enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };
 
 struct KernelTable {
-  // [op_a][op_b][op_c][mode][level][splitk][scaled]
-  KernelEntry shgemm[2][2][2][2][2][2][2];
+  // [op_a][op_b][op_c][mode][level][splitk]
+  KernelEntry shgemm[2][2][2][kNumModes][2][2];
   // [op_a][op_b][op_c]
   KernelEntry sgemm[2][2][2];
+  // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,
+  // timed by the tile sweep and selectable with MTK_SHGEMM_TUNE_P1=<index>.
+  static constexpr int kMaxTune = 16;
+  KernelEntry tune[kMaxTune];
+  int ntune = 0;
 };
 
 void register_shgemm_fp16_nn(KernelTable&);
@@ -30,7 +36,12 @@ void register_shgemm_bf16_nn(KernelTable&);
 void register_shgemm_bf16_nt(KernelTable&);
 void register_shgemm_bf16_tn(KernelTable&);
 void register_shgemm_bf16_tt(KernelTable&);
+void register_shgemm_fp16s_nn(KernelTable&);
+void register_shgemm_fp16s_nt(KernelTable&);
+void register_shgemm_fp16s_tn(KernelTable&);
+void register_shgemm_fp16s_tt(KernelTable&);
 void register_sgemm(KernelTable&);
+void register_tune(KernelTable&);
 void register_all(KernelTable&);
 
 // Auxiliary kernels (defined in aux_kernels.hip).

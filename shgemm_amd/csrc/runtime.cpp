@@ -29,7 +29,8 @@ struct handle_state {
   uint32_t* row_max = nullptr;  // prescale scratch
   int* row_exp = nullptr;
   size_t row_cap = 0;
-  bool exp_scaling = false;
+  bool exp_scaling = true;
+  int tune_p1 = -1;  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
 };
 
 }  // namespace detail
@@ -117,8 +118,7 @@ void create(shgemmHandle_t& handle) {
           for (auto& d : c)
             for (auto& e : d)
               for (auto& f : e)
-                for (auto& g : f)
-                  if (g.fn) set_lds_limit(g);
+                if (f.fn) set_lds_limit(f);
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
     st->ws_bytes = kWorkspaceBytes;
   } catch (...) {
@@ -126,8 +126,16 @@ void create(shgemmHandle_t& handle) {
     delete st;
     throw;
   }
+  const char* tp = std::getenv("MTK_SHGEMM_TUNE_P1");
+  if (tp != nullptr && tp[0] >= '0' && tp[0] <= '9') {
+    const int i = std::atoi(tp);
+    if (i < st->table.ntune) st->tune_p1 = i;
+  }
+  for (int i = 0; i < st->table.ntune; ++i) set_lds_limit(st->table.tune[i]);
+  // Per-row exponent prescaling of the fp16 compute type is on unless
+  // MTK_SHGEMM_EXP_SCALING=0 (then fp16 has FP16's exponent range, as in the reference).
   const char* sc = std::getenv("MTK_SHGEMM_EXP_SCALING");
-  st->exp_scaling = (sc != nullptr && sc[0] == '1');
+  st->exp_scaling = !(sc != nullptr && sc[0] == '0');
   handle.state = st;
 }
 
@@ -179,11 +187,13 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
     return detail::P0;
   }
 
-  const int mode = compute_type == fp16 ? shgemm_amd::MODE_FP16 : shgemm_amd::MODE_BF16;
+  const int mode = compute_type == tf32 ? shgemm_amd::MODE_BF16
+                   : st->exp_scaling ? shgemm_amd::MODE_FP16S
+                                     : shgemm_amd::MODE_FP16;
   const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
-  const int scaled = (mode == shgemm_amd::MODE_FP16 && st->exp_scaling) ? 1 : 0;
+  const bool scaled = (mode == shgemm_amd::MODE_FP16S);
 
   shgemm_amd::GemmParams p{};
   p.m = (int)m;
@@ -226,7 +236,7 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   // (reference shgemm.cu:532 uses m*n <= 884,736 && k >= 4096 && beta == 0;
   // here beta is applied by the reduction, so beta != 0 is allowed too).
   {
-    const auto& e = tab[shgemm_amd::LEVEL_P1][1][scaled];
+    const auto& e = tab[shgemm_amd::LEVEL_P1][1];
     const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
     if (tiles < ncu && k >= 4096) {
       // Slices: enough workgroups for two per CU, each slice at least 8 k-tiles,
@@ -262,12 +272,14 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   if (handle.fixed_lernel_level < detail::num_levels) {
     level = handle.fixed_lernel_level;
   } else {
-    const auto& e = tab[shgemm_amd::LEVEL_P1][0][scaled];
+    const auto& e = tab[shgemm_amd::LEVEL_P1][0];
     const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
     const int64_t filling = 2 * (int64_t)ncu;  // two workgroups per CU
     level = tiles >= 2 * filling ? detail::P1 : detail::P0;
   }
-  const auto& e = tab[level][0][scaled];
+  const bool tuned = st->tune_p1 >= 0 && level == detail::P1 && st->table.tune[st->tune_p1].mode == mode &&
+                     oa == shgemm_amd::OP_N && ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N;
+  const auto& e = tuned ? st->table.tune[st->tune_p1] : tab[level][0];
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
   p.group_m = group_rows(p.tiles_m);
@@ -338,6 +350,11 @@ void register_all(KernelTable& t) {
   register_shgemm_bf16_nt(t);
   register_shgemm_bf16_tn(t);
   register_shgemm_bf16_tt(t);
+  register_shgemm_fp16s_nn(t);
+  register_shgemm_fp16s_nt(t);
+  register_shgemm_fp16s_tn(t);
+  register_shgemm_fp16s_tt(t);
   register_sgemm(t);
+  register_tune(t);
 }
 }  // namespace shgemm_amd

@@ -18,6 +18,7 @@
 //   * epilogue: C = alpha * (hi + 2^-11 lo) + beta * C, or the slice partial
 //     into the split-K workspace.
 #pragma once
+#include <type_traits>
 #include "gemm_common.hpp"
 
 namespace shgemm_amd {
@@ -113,7 +114,9 @@ struct TileF32 {
     }
   }
 
-  // FP16 hi / lo split into two LDS arrays of R*KC chunks.
+  // FP16 hi / lo split into two LDS arrays of R*KC chunks.  SCALED: the row
+  // prescale is applied first and lo is the unscaled residual (MODE_FP16S);
+  // otherwise lo is the 2^11-scaled residual (MODE_FP16).
   template <bool SCALED>
   __device__ __forceinline__ void store_f16(uint4* __restrict__ hi, uint4* __restrict__ lo, int tid) const {
 #pragma unroll
@@ -126,7 +129,8 @@ struct TileF32 {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           _Float16 hh, ll;
-          split_f16(SCALED ? v[p][e] * scale[p][0] : v[p][e], hh, ll);
+          if constexpr (SCALED) split_f16_same_scale(v[p][e] * scale[p][0], hh, ll);
+          else split_f16(v[p][e], hh, ll);
           h[e] = hh;
           l[e] = ll;
         }
@@ -141,7 +145,8 @@ struct TileF32 {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             _Float16 hh, ll;
-            split_f16(SCALED ? v[p][4 * j + e] * scale[p][e] : v[p][4 * j + e], hh, ll);
+            if constexpr (SCALED) split_f16_same_scale(v[p][4 * j + e] * scale[p][e], hh, ll);
+            else split_f16(v[p][4 * j + e], hh, ll);
             h[j] = hh;
             l[j] = ll;
           }
@@ -339,15 +344,19 @@ struct TileF16 {
 // ---------------------------------------------------------------------------
 // Tile configuration
 // ---------------------------------------------------------------------------
-template <int BM_, int BN_, int BK_, int WM_, int WN_>
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int PF_, int OCC_>
 struct TileCfg {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+  // PF: k-tiles of global loads in flight ahead of the one being multiplied
+  // (register sets); OCC: workgroups per CU the register budget targets.
+  static constexpr int PF = PF_, OCC = OCC_;
   static constexpr int KC = BK / 8;
   static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
   static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 16 == 0, "MFMA 32x32x16 granularity");
+  static_assert(PF == 1 || PF == 2, "one or two k-tiles of prefetch");
   template <int MODE>
   static constexpr int stage_chunks() {
-    return (MODE == MODE_FP16 ? 2 : 3) * BM * KC + (MODE == MODE_FP16 ? 1 : 2) * BN * KC;
+    return (MODE == MODE_BF16 ? 3 : 2) * BM * KC + (MODE == MODE_BF16 ? 2 : 1) * BN * KC;
   }
   template <int MODE>
   static constexpr size_t lds_bytes() {
@@ -355,22 +364,39 @@ struct TileCfg {
   }
 };
 
-// P1: large tiles; P0: small tiles (the reference's two kernel levels,
-// include/shgemm/shgemm.hpp:30-34).
-using CfgP1 = TileCfg<128, 128, 32, 64, 64>;
-using CfgP0 = TileCfg<64, 64, 64, 32, 32>;
+// Tile configurations per split mode (the reference's two kernel levels,
+// include/shgemm/shgemm.hpp:30-34, plus the split-K tile):
+//   P1: large tiles; P0: small tiles for shapes with too few large tiles;
+//   SK: split-K (tall-skinny shapes have small m and n).
+// The prescaled mode has one accumulator, so its 64 x 128 wave tiles fit two
+// workgroups per CU; the two-accumulator modes use 64 x 64 wave tiles.
+// Chosen by the tile sweep at m = n = k = 8192 and 16384.
+template <int MODE>
+struct ModeCfg {
+  using P1 = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+  using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
+  using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+};
+template <>
+struct ModeCfg<MODE_FP16S> {
+  using P1 = TileCfg<128, 256, 32, 64, 128, 1, 2>;
+  using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
+  using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+};
 
 // ---------------------------------------------------------------------------
 // The kernel.
 //   OPA, OPB: operand storage (OP_N / OP_T), OPC: C storage.
-//   SCALED: A rows are prescaled by 2^-row_exp (fp16 mode only).
+//   MODE: split scheme (MODE_FP16S prescales A rows by 2^-row_exp).
 //   SPLITK: blockIdx.y selects a k-slice; the partial goes to p.ws.
 // ---------------------------------------------------------------------------
-template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK, bool SCALED>
-__global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
+template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
+__global__ void __launch_bounds__(256, CFG::OCC) shgemm_kernel(const GemmParams p) {
+  constexpr bool SCALED = (MODE == MODE_FP16S);
+  constexpr int NACC = (MODE == MODE_FP16) ? 2 : 1;  // separate hi / lo accumulators
   constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC;
   constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM;
-  constexpr int NA = MODE == MODE_FP16 ? 2 : 3;
+  constexpr int NA = MODE == MODE_BF16 ? 3 : 2;
   constexpr int A_CH = BM * KC, B_CH = BN * KC;
   constexpr int STAGE = CFG::template stage_chunks<MODE>();
   // With C column-major the MFMA's lane index runs along m (coalesced C
@@ -378,8 +404,11 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
   constexpr bool LANE_M = (OPC == OP_N);
   extern __shared__ uint4 lds[];
 
-  TileF32<OPA == OP_T, BM, BK> ta;
-  TileF16<OPB == OP_N, BN, BK> tb;
+  constexpr int PF = CFG::PF;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, PF - 1>;
+  TileF32<OPA == OP_T, BM, BK> ta[PF];
+  TileF16<OPB == OP_N, BN, BK> tb[PF];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
@@ -398,11 +427,14 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
   const bool a_full = p.vec_a && (m0 + BM <= p.m);
   const bool b_full = p.vec_b && (n0 + BN <= p.n);
 
-  if constexpr (SCALED) ta.init_scale(p.row_exp, m0, p.m, tid);
-
-  f32x16 acc[2][FM][FN];
+  if constexpr (SCALED) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+    for (int i = 0; i < PF; ++i) ta[i].init_scale(p.row_exp, m0, p.m, tid);
+  }
+
+  f32x16 acc[NACC][FM][FN];
+#pragma unroll
+  for (int h = 0; h < NACC; ++h)
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -410,29 +442,33 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[h][i][j][e] = 0.0f;
 
-  auto gload = [&](int kt) {
+  // `set` is a std::integral_constant so the staging registers are never
+  // indexed dynamically (which would force them into scratch memory).
+  auto gload = [&](auto set_c, int kt) {
+    constexpr int set = decltype(set_c)::value;
     const int k0 = kbeg + kt * BK;
     const bool kf = (k0 + BK <= kend);
-    if (a_full && kf) ta.load_fast(A, p.lda, m0, k0, tid);
-    else ta.load_safe(A, p.lda, m0, k0, p.m, kend, tid);
-    if (b_full && kf) tb.load_fast(B, p.ldb, n0, k0, tid);
-    else tb.load_safe(B, p.ldb, n0, k0, p.n, kend, tid);
+    if (a_full && kf) ta[set].load_fast(A, p.lda, m0, k0, tid);
+    else ta[set].load_safe(A, p.lda, m0, k0, p.m, kend, tid);
+    if (b_full && kf) tb[set].load_fast(B, p.ldb, n0, k0, tid);
+    else tb[set].load_safe(B, p.ldb, n0, k0, p.n, kend, tid);
   };
-  auto sstore = [&](int stage) {
+  auto sstore = [&](auto set_c, int stage) {
+    constexpr int set = decltype(set_c)::value;
     uint4* st = lds + stage * STAGE;
-    if constexpr (MODE == MODE_FP16) {
-      ta.template store_f16<SCALED>(st, st + A_CH, tid);
-      tb.store_raw(st + NA * A_CH, tid);
+    if constexpr (MODE != MODE_BF16) {
+      ta[set].template store_f16<SCALED>(st, st + A_CH, tid);
+      tb[set].store_raw(st + NA * A_CH, tid);
     } else {
-      ta.store_bf16(st, st + A_CH, st + 2 * A_CH, tid);
-      tb.store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
+      ta[set].store_bf16(st, st + A_CH, st + 2 * A_CH, tid);
+      tb[set].store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
     }
   };
   auto compute = [&](const uint4* __restrict__ st) {
 #pragma unroll
     for (int s = 0; s < BK / 16; ++s) {
       const int c = 2 * s + lh;
-      if constexpr (MODE == MODE_FP16) {
+      if constexpr (MODE != MODE_BF16) {
         f16x8 ah[FM], al[FM], bb[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
@@ -449,10 +485,10 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
           for (int j = 0; j < FN; ++j) {
             if constexpr (LANE_M) {
               acc[0][i][j] = mfma_f16(bb[j], ah[i], acc[0][i][j]);
-              acc[1][i][j] = mfma_f16(bb[j], al[i], acc[1][i][j]);
+              acc[NACC - 1][i][j] = mfma_f16(bb[j], al[i], acc[NACC - 1][i][j]);
             } else {
               acc[0][i][j] = mfma_f16(ah[i], bb[j], acc[0][i][j]);
-              acc[1][i][j] = mfma_f16(al[i], bb[j], acc[1][i][j]);
+              acc[NACC - 1][i][j] = mfma_f16(al[i], bb[j], acc[NACC - 1][i][j]);
             }
           }
       } else {
@@ -474,44 +510,68 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            // acc0: a0*b0; acc1: a0*b1 + a1*b0 + a1*b1 + a2*b0 (a2*b1 ~ 2^-24 dropped)
+            // a0*b0 + a0*b1 + a1*b0 + a1*b1 + a2*b0 (a2*b1 ~ 2^-24 dropped), all in
+            // one accumulator: each MFMA adds its exact product sum with one
+            // round-to-nearest-even.
+            f32x16& c = acc[0][i][j];
             if constexpr (LANE_M) {
-              acc[0][i][j] = mfma_bf16(b0[j], a0[i], acc[0][i][j]);
-              acc[1][i][j] = mfma_bf16(b1[j], a0[i], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(b0[j], a1[i], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(b1[j], a1[i], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(b0[j], a2[i], acc[1][i][j]);
+              c = mfma_bf16(b0[j], a0[i], c);
+              c = mfma_bf16(b1[j], a0[i], c);
+              c = mfma_bf16(b0[j], a1[i], c);
+              c = mfma_bf16(b1[j], a1[i], c);
+              c = mfma_bf16(b0[j], a2[i], c);
             } else {
-              acc[0][i][j] = mfma_bf16(a0[i], b0[j], acc[0][i][j]);
-              acc[1][i][j] = mfma_bf16(a0[i], b1[j], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(a1[i], b0[j], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(a1[i], b1[j], acc[1][i][j]);
-              acc[1][i][j] = mfma_bf16(a2[i], b0[j], acc[1][i][j]);
+              c = mfma_bf16(a0[i], b0[j], c);
+              c = mfma_bf16(a0[i], b1[j], c);
+              c = mfma_bf16(a1[i], b0[j], c);
+              c = mfma_bf16(a1[i], b1[j], c);
+              c = mfma_bf16(a2[i], b0[j], c);
             }
           }
       }
     }
   };
 
-  // Register-staged double buffer: one barrier per k-tile.  The store into
-  // stage cur^1 is safe because the barrier closing the previous iteration
-  // ended every read of that stage.
-  if (ktiles > 0) {
-    gload(0);
-    sstore(0);
-  }
-  __syncthreads();
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < ktiles) gload(kt + 1);
-    compute(lds + cur * STAGE);
-    if (kt + 1 < ktiles) sstore(cur ^ 1);
+  // LDS double buffer, one barrier per k-tile.  The store into stage cur^1 is
+  // safe because the barrier closing the previous iteration ended every read
+  // of that stage.  With PF == 2 the global loads of tile kt+2 are issued
+  // before tile kt is multiplied and stored to LDS only after tile kt+1 is,
+  // so each load has two k-tiles of MFMA work to land behind.
+  if constexpr (PF == 1) {
+    if (ktiles > 0) {
+      gload(S0{}, 0);
+      sstore(S0{}, 0);
+    }
     __syncthreads();
+    for (int kt = 0; kt < ktiles; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < ktiles) gload(S0{}, kt + 1);
+      compute(lds + cur * STAGE);
+      if (kt + 1 < ktiles) sstore(S0{}, cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    if (ktiles > 0) gload(S0{}, 0);
+    if (ktiles > 1) gload(S1{}, 1);
+    if (ktiles > 0) sstore(S0{}, 0);
+    __syncthreads();
+    // Register set s holds tile kt+1 (odd/even alternate), stage kt&1 holds tile kt.
+    for (int kt = 0; kt < ktiles; kt += 2) {
+      if (kt + 2 < ktiles) gload(S0{}, kt + 2);
+      compute(lds);
+      if (kt + 1 < ktiles) sstore(S1{}, 1);
+      __syncthreads();
+      if (kt + 1 >= ktiles) break;
+      if (kt + 3 < ktiles) gload(S1{}, kt + 3);
+      compute(lds + STAGE);
+      if (kt + 2 < ktiles) sstore(S0{}, 0);
+      __syncthreads();
+    }
   }
 
   // Epilogue.  Lane/register -> (m, n) of the 32x32 MFMA output block:
   // lane index gives the column, register j gives row (j/4)*8 + (lane/32)*4 + j%4.
-  constexpr float lo_w = MODE == MODE_FP16 ? (1.0f / 2048.0f) : 1.0f;
+  constexpr float lo_w = 1.0f / 2048.0f;
   const float alpha = p.alpha, beta = p.beta;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -529,7 +589,8 @@ __global__ void __launch_bounds__(256, 2) shgemm_kernel(const GemmParams p) {
           nn = n0 + wn * WN + j * 32 + lrow;
         }
         if (mm < p.m && nn < p.n) {
-          float v = __builtin_fmaf(acc[1][i][j][e], lo_w, acc[0][i][j][e]);
+          float v = acc[0][i][j][e];
+          if constexpr (NACC == 2) v = __builtin_fmaf(acc[1][i][j][e], lo_w, v);
           if constexpr (SCALED) v = __builtin_ldexpf(v, p.row_exp[mm]);
           if constexpr (SPLITK) {
             const int64_t off = (int64_t)blockIdx.y * p.m * p.n +

@@ -25,12 +25,26 @@ def read_buffer(buf: torch.Tensor, offset: int, size: int) -> np.ndarray:
     return buf[offset:offset + size].cpu().numpy()
 
 
+# Test-side compute variants: 0 = fp16 (default: per-row prescaled, one
+# accumulator), 1 = tf32 (BF16 split), 2 = fp16 with prescaling off (the
+# reference's hi + 2^-11 lo with two accumulators).
+FP16, TF32, FP16_CLASSIC = 0, 1, 2
+VARIANTS = [FP16, TF32, FP16_CLASSIC]
+
+
 def run_shgemm(h, op_a, op_b, op_c, m, n, k, alpha, A, lda, B16, ldb, beta, C0, ldc, compute_type, off=(0, 0, 0)):
     """A float32 flat, B16 float16 flat, C0 float32 flat (C's storage).  Returns (C, level)."""
     a_t, a_p = dev_buffer(A, off[0])
     b_t, b_p = dev_buffer(B16.view(np.float16) if B16.dtype == np.uint16 else B16, off[1])
     c_t, c_p = dev_buffer(C0, off[2])
-    lvl = sg.shgemm(h, op_a, op_b, m, n, k, alpha, a_p, lda, b_p, ldb, beta, c_p, ldc, compute_type, op_c)
+    ct = 0 if compute_type == FP16_CLASSIC else compute_type
+    if compute_type == FP16_CLASSIC:
+        sg.set_exponent_scaling(h, False)
+    try:
+        lvl = sg.shgemm(h, op_a, op_b, m, n, k, alpha, a_p, lda, b_p, ldb, beta, c_p, ldc, ct, op_c)
+    finally:
+        if compute_type == FP16_CLASSIC:
+            sg.set_exponent_scaling(h, True)
     return read_buffer(c_t, off[2], C0.size), lvl
 
 

@@ -18,12 +18,11 @@ import pytest
 
 import oracle
 from tests.golden.make_golden import load_cases
-from tests.gpu_util import lds, require_gpu, run_hsgemm, run_sgemm, run_shgemm, sizes
+from tests.gpu_util import FP16, FP16_CLASSIC, TF32, VARIANTS, lds, require_gpu, run_hsgemm, run_sgemm, run_shgemm, sizes
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-FP16, TF32 = 0, 1
 
 
 @pytest.fixture(scope="module")
@@ -47,7 +46,7 @@ def _sgemm_err(h, c):
     return _err(Cs, c["C64"], c["op_c"], c["m"], c["n"], c["ldc"])
 
 
-@pytest.mark.parametrize("compute_type", [FP16, TF32])
+@pytest.mark.parametrize("compute_type", VARIANTS)
 def test_golden_integer_bit_exact(h, compute_type):
     for c in load_cases(GOLDEN / "gemm_int.npz"):
         C, _ = run_shgemm(h, c["op_a"], c["op_b"], c["op_c"], c["m"], c["n"], c["k"], c["alpha"], c["A"], c["lda"],
@@ -57,7 +56,7 @@ def test_golden_integer_bit_exact(h, compute_type):
         assert np.array_equal(view.astype(np.float64), ref), (c["op_a"], c["op_b"], c["op_c"], c["m"], c["n"], c["k"])
 
 
-@pytest.mark.parametrize("compute_type", [FP16, TF32])
+@pytest.mark.parametrize("compute_type", VARIANTS)
 def test_golden_random_vs_fp64(h, compute_type):
     for c in load_cases(GOLDEN / "gemm_rand.npz"):
         C, _ = run_shgemm(h, c["op_a"], c["op_b"], c["op_c"], c["m"], c["n"], c["k"], c["alpha"], c["A"], c["lda"],
@@ -98,7 +97,7 @@ def _run_s(h, c):
                      c["B32"], c["ldb"], c["beta"], c["C0"], c["ldc"])
 
 
-@pytest.mark.parametrize("compute_type", [FP16, TF32])
+@pytest.mark.parametrize("compute_type", VARIANTS)
 def test_cfg1_1024_nn_within_2x_sgemm(h, compute_type):
     """1024^3 op_n/op_n, A U[-1,1] FP32, B U[-1,1] FP16, alpha=1, beta=0."""
     rng = np.random.default_rng(1)
@@ -113,11 +112,11 @@ def test_cfg1_1024_nn_within_2x_sgemm(h, compute_type):
     assert err <= 2.0 * err_s, (err, err_s)
     # the kernel multiplies exactly the split representation: FP32-accumulation agreement
     emu = oracle.emulate_shgemm(0, 0, 0, 1024, 1024, 1024, 1.0, c["A"], 1024, c["B32"], 1024, 0.0, None, 1024,
-                                compute_type)
+                                {FP16: 2, TF32: 1, FP16_CLASSIC: 0}[compute_type])
     assert _err(C, emu, 0, 1024, 1024, 1024) <= 2.0 * err_s
 
 
-@pytest.mark.parametrize("compute_type", [FP16, TF32])
+@pytest.mark.parametrize("compute_type", VARIANTS)
 @pytest.mark.parametrize("op_a", [0, 1])
 @pytest.mark.parametrize("op_b", [0, 1])
 @pytest.mark.parametrize("op_c", [0, 1])

@@ -35,8 +35,9 @@ def _fp64_nn(A, B32, m, n, k):
     return Am @ Bm
 
 
+@pytest.mark.parametrize("scaling", [True, False])
 @pytest.mark.parametrize("k", [1 << 16, 1 << 20])
-def test_splitk_tall_skinny_bit_identical(h, k):
+def test_splitk_tall_skinny_bit_identical(h, k, scaling):
     """m = n = 128, k up to 2^20 (A U[-1,1], B N(0,1) FP16), op_n/op_n."""
     import torch
 
@@ -51,10 +52,14 @@ def test_splitk_tall_skinny_bit_identical(h, k):
     b_t, b_p = dev_buffer(B16)
     c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
     runs = []
-    for _ in range(10):
-        c_t.fill_(float("nan"))
-        sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, FP16)
-        runs.append(read_buffer(c_t, 0, m * n).copy())
+    sg.set_exponent_scaling(h, scaling)
+    try:
+        for _ in range(10):
+            c_t.fill_(float("nan"))
+            sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, FP16)
+            runs.append(read_buffer(c_t, 0, m * n).copy())
+    finally:
+        sg.set_exponent_scaling(h, True)
     for r in runs[1:]:
         assert np.array_equal(r.view(np.uint32), runs[0].view(np.uint32))
     ref = _fp64_nn(A, B32, m, n, k)
@@ -108,16 +113,16 @@ def test_wide_exponent_fp16_needs_prescale(h):
     Cs = run_sgemm(h, 0, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, np.zeros(m * n, np.float32), m).reshape(n, m).T
     err_s = oracle.rel_fro_error(Cs, ref)
     # fp16 without prescaling: FP16 overflow of hi, as in the reference's fp16 mode
-    sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, FP16)
-    raw = read_buffer(c_t, 0, m * n)
-    assert not np.isfinite(raw).all()
-    # per-row prescaling
-    sg.set_exponent_scaling(h, True)
+    sg.set_exponent_scaling(h, False)
     try:
         sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, FP16)
-        got = read_buffer(c_t, 0, m * n).reshape(n, m).T
+        raw = read_buffer(c_t, 0, m * n)
     finally:
-        sg.set_exponent_scaling(h, False)
+        sg.set_exponent_scaling(h, True)
+    assert not np.isfinite(raw).all()
+    # per-row prescaling (the default)
+    sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, FP16)
+    got = read_buffer(c_t, 0, m * n).reshape(n, m).T
     err = oracle.rel_fro_error(got, ref)
     print(f"wide exponent fp16+prescale err={err:.3e} sgemm_err={err_s:.3e}")
     assert err <= 2.0 * err_s
@@ -152,13 +157,9 @@ def test_wide_exponent_hsgemm_transposed_roles(h):
     a_t, a_p = dev_buffer(A)        # A stored m x k column-major; A^T is op_t of it
     b_t, b_p = dev_buffer(B16)      # Omega stored k x n; Omega^T is op_t of it
     c_t, c_p = dev_buffer(np.zeros(n * m, np.float32))
-    sg.set_exponent_scaling(h, True)
-    try:
-        # hsgemm(op_a=t on Omega, op_b=t on A): C (n x m) = Omega^T A^T
-        sg.hsgemm(h, 1, 1, n, m, k, 1.0, b_p, k, a_p, m, 0.0, c_p, n, FP16)
-        got = read_buffer(c_t, 0, n * m).reshape(m, n).T      # n x m col-major
-    finally:
-        sg.set_exponent_scaling(h, False)
+    # hsgemm(op_a=t on Omega, op_b=t on A): C (n x m) = Omega^T A^T
+    sg.hsgemm(h, 1, 1, n, m, k, 1.0, b_p, k, a_p, m, 0.0, c_p, n, FP16)
+    got = read_buffer(c_t, 0, n * m).reshape(m, n).T      # n x m col-major
     ref = (A.reshape(k, m).T.astype(np.float64) @ B32.reshape(n, k).T.astype(np.float64)).T
     s = (A.reshape(k, m).T @ B32.reshape(n, k).T).T.astype(np.float64)
     err, err_s = oracle.rel_fro_error(got, ref), oracle.rel_fro_error(s, ref)

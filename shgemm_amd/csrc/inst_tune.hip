@@ -11,16 +11,20 @@ static void put_tune(KernelTable& t) {
   e.bn = CFG::BN;
   e.bk = CFG::BK;
   e.mode = MODE;
+  e.threads = CFG::NT;
 }
 void register_tune(KernelTable& t) {
   t.ntune = 0;
-  put_tune<TileCfg<128, 128, 32, 64, 64, 1, 2>, MODE_FP16>(t);
-  put_tune<TileCfg<128, 128, 32, 64, 64, 1, 2>, MODE_FP16S>(t);
-  put_tune<TileCfg<128, 256, 32, 64, 128, 1, 2>, MODE_FP16S>(t);
-  put_tune<TileCfg<128, 256, 32, 64, 128, 1, 1>, MODE_FP16S>(t);
-  put_tune<TileCfg<256, 128, 32, 128, 64, 1, 2>, MODE_FP16S>(t);
-  put_tune<TileCfg<128, 128, 32, 64, 64, 2, 2>, MODE_FP16S>(t);
-  put_tune<TileCfg<256, 256, 32, 128, 128, 1, 1>, MODE_FP16S>(t);
-  put_tune<TileCfg<128, 256, 64, 64, 128, 1, 1>, MODE_FP16S>(t);
+  put_tune<TileCfg<128, 128, 32, 64, 64, 1, 2>, MODE_FP16>(t);     // 0
+  put_tune<TileCfg<128, 256, 32, 64, 128, 1, 2>, MODE_FP16S>(t);   // 1 (default P1)
+  put_tune<TileCfg<128, 128, 32, 64, 64, 1, 3>, MODE_FP16S>(t);    // 2
+  put_tune<TileCfg<256, 256, 32, 64, 128, 1, 1>, MODE_FP16S>(t);   // 3: 8 waves
+  put_tune<TileCfg<128, 256, 32, 64, 64, 1, 2>, MODE_FP16S>(t);    // 4: 8 waves
+  put_tune<TileCfg<256, 128, 32, 64, 64, 1, 2>, MODE_FP16S>(t);    // 5: 8 waves
+  put_tune<TileCfg<256, 128, 32, 64, 64, 1, 1>, MODE_FP16S>(t);    // 6: 8 waves
+  put_tune<TileCfg<128, 256, 64, 64, 64, 1, 1>, MODE_FP16S>(t);    // 7: 8 waves, BK 64
+  put_tune<TileCfg<256, 256, 32, 64, 64, 1, 1>, MODE_FP16S>(t);    // 8: 16 waves
+  put_tune<TileCfg<256, 256, 32, 64, 128, 1, 1>, MODE_FP16S>(t);   // 9: 8 waves, w64x128
+  put_tune<TileCfg<128, 512, 32, 64, 128, 1, 1>, MODE_FP16S>(t);   // 10: 8 waves
 }
 }  // namespace shgemm_amd

@@ -16,6 +16,7 @@ inline void put_shgemm(KernelTable& t, int level) {
   e.bn = CFG::BN;
   e.bk = CFG::BK;
   e.mode = MODE;
+  e.threads = CFG::NT;
 }
 
 template <int OPA, int OPB, int OPC, int MODE>

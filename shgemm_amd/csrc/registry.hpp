@@ -12,6 +12,7 @@ struct KernelEntry {
   size_t lds = 0;      // dynamic LDS bytes
   int bm = 0, bn = 0, bk = 0;
   int mode = -1;
+  int threads = 256;
 };
 
 enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };

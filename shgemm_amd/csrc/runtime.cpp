@@ -66,7 +66,7 @@ void set_lds_limit(const shgemm_amd::KernelEntry& e) {
 void launch(const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmParams& p, hipStream_t s) {
   if (e.fn == nullptr) throw error("internal: kernel variant not compiled");
   void* args[] = {const_cast<shgemm_amd::GemmParams*>(&p)};
-  check_hip(hipLaunchKernel(e.fn, grid, dim3(256), args, e.lds, s), "kernel launch");
+  check_hip(hipLaunchKernel(e.fn, grid, dim3(e.threads), args, e.lds, s), "kernel launch");
 }
 
 const char* op_name(operation_t op) { return op == op_n ? "N" : "T"; }

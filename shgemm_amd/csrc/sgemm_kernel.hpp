@@ -2,7 +2,7 @@
 // The comparator the error-corrected SHGEMM is measured against: same inputs
 // (B already widened to FP32), same tiling scheme, one accumulator.
 //
-// LDS: k-chunk-major FP32, chunks of 4 values (16 bytes), slot as lds_slot.
+// LDS: FP32 chunks of 4 values (16 bytes) in swizzled row-major slots (lds_slot_rm).
 // MFMA k-order: inside each 16-k group, lane half h owns k = 8h..8h+7 and
 // MFMA step s uses k = s (h=0) and 8+s (h=1); A and B use the same order, so
 // every k is multiplied exactly once.
@@ -11,23 +11,26 @@
 
 namespace shgemm_amd {
 
-// FP32 tile, chunks of 4 k.  KCONT item = (row, chunk): one 16-byte load.
-// !KCONT item = (4 rows, chunk): four 16-byte loads, transposed in registers.
+// FP32 tile, chunks of 4 k (16 bytes), row-major swizzled slots (lds_slot_rm).
+// KCONT item = (row, chunk), chunk fastest: one 16-byte load.
+// !KCONT item = (row, chunk), row fastest: four 4-byte loads down 4 columns,
+// each instruction reading 64 consecutive values of one column.
 template <bool KCONT, int R, int BK>
 struct TileF32x4 {
   static constexpr int KC = BK / 4;
-  static constexpr int NITEMS = KCONT ? R * KC : (R / 4) * KC;
+  static constexpr int NITEMS = R * KC;
   static constexpr int PER = NITEMS / 256;
   static_assert(NITEMS % 256 == 0 && PER >= 1, "tile must split evenly over 256 threads");
-  f32x4 v[PER][KCONT ? 1 : 4];
+  static_assert(KCONT || R % 64 == 0, "row-fastest map needs whole wavefronts per chunk");
+  f32x4 v[PER];
 
   __device__ __forceinline__ static void item(int i, int& r, int& c) {
     if constexpr (KCONT) {
       r = i / KC;
       c = i % KC;
     } else {
-      r = 4 * (i % (R / 4));
-      c = i / (R / 4);
+      r = i % R;
+      c = i / R;
     }
   }
 
@@ -37,29 +40,18 @@ struct TileF32x4 {
     for (int p = 0; p < PER; ++p) {
       int r, c;
       item(tid + 256 * p, r, c);
-      const int k = k0 + 4 * c;
+      const int k = k0 + 4 * c, rr = r0 + r;
       if constexpr (KCONT) {
         if (fast) {
-          v[p][0] = *reinterpret_cast<const f32x4*>(base + (int64_t)(r0 + r) * ld + k);
+          v[p] = *reinterpret_cast<const f32x4*>(base + (int64_t)rr * ld + k);
         } else {
-          const int rr = r0 + r;
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            v[p][0][e] = (rr < rlim && k + e < klim) ? base[(int64_t)rr * ld + k + e] : 0.0f;
+          for (int e = 0; e < 4; ++e) v[p][e] = (rr < rlim && k + e < klim) ? base[(int64_t)rr * ld + k + e] : 0.0f;
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (fast) {
-            v[p][j] = *reinterpret_cast<const f32x4*>(base + (r0 + r) + (int64_t)(k + j) * ld);
-          } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int rr = r0 + r + e;
-              v[p][j][e] = (rr < rlim && k + j < klim) ? base[rr + (int64_t)(k + j) * ld] : 0.0f;
-            }
-          }
-        }
+        for (int e = 0; e < 4; ++e)
+          v[p][e] = (fast || (rr < rlim && k + e < klim)) ? base[rr + (int64_t)(k + e) * ld] : 0.0f;
       }
     }
   }
@@ -69,15 +61,7 @@ struct TileF32x4 {
     for (int p = 0; p < PER; ++p) {
       int r, c;
       item(tid + 256 * p, r, c);
-      if constexpr (KCONT) {
-        *reinterpret_cast<f32x4*>(&dst[lds_slot<R>(c, r)]) = v[p][0];
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const f32x4 o{v[p][0][e], v[p][1][e], v[p][2][e], v[p][3][e]};
-          *reinterpret_cast<f32x4*>(&dst[lds_slot<R>(c, r + e)]) = o;
-        }
-      }
+      *reinterpret_cast<f32x4*>(&dst[lds_slot_rm<KC>(r, c)]) = v[p];
     }
   }
 };
@@ -144,14 +128,14 @@ __global__ void __launch_bounds__(256, 2) sgemm_kernel(const GemmParams p) {
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         const int r = wm * WM + i * 32 + lrow;
-        a[i][0] = *reinterpret_cast<const f32x4*>(&st[lds_slot<BM>(c, r)]);
-        a[i][1] = *reinterpret_cast<const f32x4*>(&st[lds_slot<BM>(c + 1, r)]);
+        a[i][0] = *reinterpret_cast<const f32x4*>(&st[lds_slot_rm<KC>(r, c)]);
+        a[i][1] = *reinterpret_cast<const f32x4*>(&st[lds_slot_rm<KC>(r, c + 1)]);
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const int r = wn * WN + j * 32 + lrow;
-        b[j][0] = *reinterpret_cast<const f32x4*>(&st[A_CH + lds_slot<BN>(c, r)]);
-        b[j][1] = *reinterpret_cast<const f32x4*>(&st[A_CH + lds_slot<BN>(c + 1, r)]);
+        b[j][0] = *reinterpret_cast<const f32x4*>(&st[A_CH + lds_slot_rm<KC>(r, c)]);
+        b[j][1] = *reinterpret_cast<const f32x4*>(&st[A_CH + lds_slot_rm<KC>(r, c + 1)]);
       }
 #pragma unroll
       for (int s = 0; s < 8; ++s)

@@ -24,52 +24,61 @@
 namespace shgemm_amd {
 
 // ---------------------------------------------------------------------------
-// FP32 operand tile: R rows x BK k-values, 256 threads.
+// Operand tiles: R rows x BK k-values, NT threads, k cut into KC = BK/8
+// chunks of 8 values (16 bytes once in FP16/BF16).  One item = (row r,
+// chunk c): the thread loads those 8 values, converts them and writes one
+// 16-byte LDS slot per part (lds_slot_rm).
 // Element (r, k) sits at base[r*ld + k] when KCONT (k contiguous) and at
-// base[r + k*ld] otherwise.
-//   KCONT item = (row r, chunk of 8 k): two 16-byte loads.
-//   !KCONT item = (4 rows, 4 k): four 16-byte loads down 4 columns.
+// base[r + k*ld] otherwise.  The item -> thread maps are chosen for both
+// coalesced global loads and conflict-free LDS stores:
+//   KCONT: chunk fastest (KC lanes share one row segment; 16- or 32-byte loads);
+//          a wavefront stores 64/KC whole rows.
+//   !KCONT: row fastest (each load instruction reads 64 consecutive values of
+//          one k column); a wavefront stores one chunk of 64 consecutive rows.
 // ---------------------------------------------------------------------------
-template <bool KCONT, int R, int BK>
-struct TileF32 {
+template <bool KCONT, int R, int BK, int NT>
+struct ItemMap {
   static constexpr int KC = BK / 8;
-  static constexpr int NITEMS = KCONT ? R * KC : (R / 4) * KC * 2;
-  static constexpr int PER = NITEMS / 256;
-  static_assert(NITEMS % 256 == 0 && PER >= 1, "tile must split evenly over 256 threads");
-  static constexpr int NV = KCONT ? 8 : 16;
-  float v[PER][NV];
-  float scale[PER][KCONT ? 1 : 4];  // per-row prescale 2^-e(row)
-
-  __device__ __forceinline__ static void item(int i, int& r, int& k) {
+  static constexpr int NITEMS = R * KC;
+  static constexpr int PER = NITEMS / NT;
+  static_assert(NITEMS % NT == 0 && PER >= 1, "tile must split evenly over the workgroup");
+  static_assert(KCONT || R % 64 == 0, "row-fastest map needs whole wavefronts per chunk");
+  __device__ __forceinline__ static void item(int i, int& r, int& c) {
     if constexpr (KCONT) {
       r = i / KC;
-      k = 8 * (i % KC);
+      c = i % KC;
     } else {
-      r = 4 * (i % (R / 4));
-      k = 4 * (i / (R / 4));
+      r = i % R;
+      c = i / R;
     }
   }
+};
+
+// FP32 operand (A of SHGEMM).
+template <bool KCONT, int R, int BK, int NT>
+struct TileF32 : ItemMap<KCONT, R, BK, NT> {
+  using M = ItemMap<KCONT, R, BK, NT>;
+  static constexpr int PER = M::PER, KC = M::KC;
+  float v[PER][8];
+  float scale[PER];  // per-row prescale 2^-e(row)
 
   __device__ __forceinline__ void init_scale(const int* row_exp, int r0, int rlim, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-#pragma unroll
-      for (int e = 0; e < (KCONT ? 1 : 4); ++e) {
-        const int rr = r0 + r + e;
-        scale[p][e] = (row_exp != nullptr && rr < rlim) ? __builtin_ldexpf(1.0f, -row_exp[rr]) : 1.0f;
-      }
+      int r, c;
+      M::item(tid + NT * p, r, c);
+      const int rr = r0 + r;
+      scale[p] = (row_exp != nullptr && rr < rlim) ? __builtin_ldexpf(1.0f, -row_exp[rr]) : 1.0f;
     }
   }
 
   __device__ __forceinline__ void load_fast(const float* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
+      int r, c;
+      M::item(tid + NT * p, r, c);
       if constexpr (KCONT) {
-        const float* ptr = base + (int64_t)(r0 + r) * ld + (k0 + k);
+        const float* ptr = base + (int64_t)(r0 + r) * ld + (k0 + 8 * c);
         const f32x4 x = *reinterpret_cast<const f32x4*>(ptr);
         const f32x4 y = *reinterpret_cast<const f32x4*>(ptr + 4);
 #pragma unroll
@@ -78,12 +87,9 @@ struct TileF32 {
           v[p][4 + e] = y[e];
         }
       } else {
+        const float* ptr = base + (r0 + r) + (int64_t)(k0 + 8 * c) * ld;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const f32x4 x = *reinterpret_cast<const f32x4*>(base + (r0 + r) + (int64_t)(k0 + k + j) * ld);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) v[p][4 * j + e] = x[e];
-        }
+        for (int e = 0; e < 8; ++e) v[p][e] = ptr[(int64_t)e * ld];
       }
     }
   }
@@ -92,69 +98,39 @@ struct TileF32 {
                                             int rlim, int klim, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-      if constexpr (KCONT) {
-        const int rr = r0 + r;
+      int r, c;
+      M::item(tid + NT * p, r, c);
+      const int rr = r0 + r;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int kk = k0 + k + e;
-          v[p][e] = (rr < rlim && kk < klim) ? base[(int64_t)rr * ld + kk] : 0.0f;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int rr = r0 + r + e, kk = k0 + k + j;
-            v[p][4 * j + e] = (rr < rlim && kk < klim) ? base[rr + (int64_t)kk * ld] : 0.0f;
-          }
-        }
+      for (int e = 0; e < 8; ++e) {
+        const int kk = k0 + 8 * c + e;
+        const bool in = rr < rlim && kk < klim;
+        v[p][e] = in ? (KCONT ? base[(int64_t)rr * ld + kk] : base[rr + (int64_t)kk * ld]) : 0.0f;
       }
     }
   }
 
-  // FP16 hi / lo split into two LDS arrays of R*KC chunks.  SCALED: the row
-  // prescale is applied first and lo is the unscaled residual (MODE_FP16S);
-  // otherwise lo is the 2^11-scaled residual (MODE_FP16).
+  // FP16 hi / lo into two LDS arrays of R*KC slots.  SCALED: the row prescale
+  // is applied first and lo is the unscaled residual (MODE_FP16S); otherwise
+  // lo is the 2^11-scaled residual (MODE_FP16).
   template <bool SCALED>
   __device__ __forceinline__ void store_f16(uint4* __restrict__ hi, uint4* __restrict__ lo, int tid) const {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-      const int c = k >> 3;
-      if constexpr (KCONT) {
-        f16x8 h, l;
+      int r, c;
+      M::item(tid + NT * p, r, c);
+      f16x8 h, l;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          _Float16 hh, ll;
-          if constexpr (SCALED) split_f16_same_scale(v[p][e] * scale[p][0], hh, ll);
-          else split_f16(v[p][e], hh, ll);
-          h[e] = hh;
-          l[e] = ll;
-        }
-        const int s = lds_slot<R>(c, r);
-        *reinterpret_cast<f16x8*>(&hi[s]) = h;
-        *reinterpret_cast<f16x8*>(&lo[s]) = l;
-      } else {
-        const int kh = (k >> 2) & 1;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          f16x4 h, l;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            _Float16 hh, ll;
-            if constexpr (SCALED) split_f16_same_scale(v[p][4 * j + e] * scale[p][e], hh, ll);
-            else split_f16(v[p][4 * j + e], hh, ll);
-            h[j] = hh;
-            l[j] = ll;
-          }
-          const int s = lds_slot<R>(c, r + e);
-          reinterpret_cast<f16x4*>(&hi[s])[kh] = h;
-          reinterpret_cast<f16x4*>(&lo[s])[kh] = l;
-        }
+      for (int e = 0; e < 8; ++e) {
+        _Float16 hh, ll;
+        if constexpr (SCALED) split_f16_same_scale(v[p][e] * scale[p], hh, ll);
+        else split_f16(v[p][e], hh, ll);
+        h[e] = hh;
+        l[e] = ll;
       }
+      const int s = lds_slot_rm<KC>(r, c);
+      *reinterpret_cast<f16x8*>(&hi[s]) = h;
+      *reinterpret_cast<f16x8*>(&lo[s]) = l;
     }
   }
 
@@ -163,82 +139,61 @@ struct TileF32 {
                                              uint4* __restrict__ d2, int tid) const {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-      const int c = k >> 3;
-      if constexpr (KCONT) {
-        bf16x8 x0, x1, x2;
+      int r, c;
+      M::item(tid + NT * p, r, c);
+      bf16x8 x0, x1, x2;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          __bf16 a0, a1, a2;
-          split_bf16x3(v[p][e], a0, a1, a2);
-          x0[e] = a0;
-          x1[e] = a1;
-          x2[e] = a2;
-        }
-        const int s = lds_slot<R>(c, r);
-        *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
-        *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
-        *reinterpret_cast<bf16x8*>(&d2[s]) = x2;
-      } else {
-        const int kh = (k >> 2) & 1;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          bf16x4 x0, x1, x2;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            __bf16 a0, a1, a2;
-            split_bf16x3(v[p][4 * j + e], a0, a1, a2);
-            x0[j] = a0;
-            x1[j] = a1;
-            x2[j] = a2;
-          }
-          const int s = lds_slot<R>(c, r + e);
-          reinterpret_cast<bf16x4*>(&d0[s])[kh] = x0;
-          reinterpret_cast<bf16x4*>(&d1[s])[kh] = x1;
-          reinterpret_cast<bf16x4*>(&d2[s])[kh] = x2;
-        }
+      for (int e = 0; e < 8; ++e) {
+        __bf16 a0, a1, a2;
+        split_bf16x3(v[p][e], a0, a1, a2);
+        x0[e] = a0;
+        x1[e] = a1;
+        x2[e] = a2;
       }
+      const int s = lds_slot_rm<KC>(r, c);
+      *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
+      *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
+      *reinterpret_cast<bf16x8*>(&d2[s]) = x2;
     }
   }
 };
 
-// ---------------------------------------------------------------------------
-// FP16 operand tile (raw binary16 bits), same item scheme as TileF32.
-//   KCONT item = (row, chunk of 8 k): one 16-byte load.
-//   !KCONT item = (4 rows, 4 k): four 8-byte loads, transposed in registers.
-// ---------------------------------------------------------------------------
-template <bool KCONT, int R, int BK>
+// FP16 operand (B of SHGEMM), raw binary16 bits.
+//   KCONT: one 16-byte load per item.
+//   !KCONT: the item covers 2 rows; eight 4-byte loads (2 rows at each k),
+//           unpacked into two 16-byte slots.
+template <bool KCONT, int R, int BK, int NT>
 struct TileF16 {
   static constexpr int KC = BK / 8;
-  static constexpr int NITEMS = KCONT ? R * KC : (R / 4) * KC * 2;
-  static constexpr int PER = NITEMS / 256;
-  static_assert(NITEMS % 256 == 0 && PER >= 1, "tile must split evenly over 256 threads");
-  // KCONT: v[p][0] holds 8 halves; !KCONT: v[p][j] (j<4) holds rows r..r+3 at k+j
+  static constexpr int ROWS_PER_ITEM = KCONT ? 1 : 2;
+  static constexpr int NITEMS = R * KC / ROWS_PER_ITEM;
+  static constexpr int PER = NITEMS / NT;
+  static_assert(NITEMS % NT == 0 && PER >= 1, "tile must split evenly over the workgroup");
+  // KCONT: v[p] holds 8 halves of one row; !KCONT: v[p][e] holds rows r, r+1 at k0+e
   u32x4 vk[KCONT ? PER : 1];
-  u32x2 vr[KCONT ? 1 : PER][4];
+  uint32_t vr[KCONT ? 1 : PER][8];
 
-  __device__ __forceinline__ static void item(int i, int& r, int& k) {
+  __device__ __forceinline__ static void item(int i, int& r, int& c) {
     if constexpr (KCONT) {
       r = i / KC;
-      k = 8 * (i % KC);
+      c = i % KC;
     } else {
-      r = 4 * (i % (R / 4));
-      k = 4 * (i / (R / 4));
+      r = 2 * (i % (R / 2));
+      c = i / (R / 2);
     }
   }
 
   __device__ __forceinline__ void load_fast(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
+      int r, c;
+      item(tid + NT * p, r, c);
       if constexpr (KCONT) {
-        vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)(r0 + r) * ld + (k0 + k));
+        vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)(r0 + r) * ld + (k0 + 8 * c));
       } else {
+        const uint16_t* ptr = base + (r0 + r) + (int64_t)(k0 + 8 * c) * ld;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          vr[p][j] = *reinterpret_cast<const u32x2*>(base + (r0 + r) + (int64_t)(k0 + k + j) * ld);
+        for (int e = 0; e < 8; ++e) vr[p][e] = *reinterpret_cast<const uint32_t*>(ptr + (int64_t)e * ld);
       }
     }
   }
@@ -247,14 +202,14 @@ struct TileF16 {
                                             int rlim, int klim, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
+      int r, c;
+      item(tid + NT * p, r, c);
       if constexpr (KCONT) {
         const int rr = r0 + r;
         uint32_t w[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int kk = k0 + k + 2 * q;
+          const int kk = k0 + 8 * c + 2 * q;
           const uint32_t x0 = (rr < rlim && kk < klim) ? base[(int64_t)rr * ld + kk] : 0u;
           const uint32_t x1 = (rr < rlim && kk + 1 < klim) ? base[(int64_t)rr * ld + kk + 1] : 0u;
           w[q] = x0 | (x1 << 16);
@@ -262,80 +217,65 @@ struct TileF16 {
         vk[p] = u32x4{w[0], w[1], w[2], w[3]};
       } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int kk = k0 + k + j;
-          uint32_t x[4];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int rr = r0 + r + e;
-            x[e] = (rr < rlim && kk < klim) ? base[rr + (int64_t)kk * ld] : 0u;
-          }
-          vr[p][j] = u32x2{x[0] | (x[1] << 16), x[2] | (x[3] << 16)};
+        for (int e = 0; e < 8; ++e) {
+          const int kk = k0 + 8 * c + e, rr = r0 + r;
+          const uint32_t x0 = (rr < rlim && kk < klim) ? base[rr + (int64_t)kk * ld] : 0u;
+          const uint32_t x1 = (rr + 1 < rlim && kk < klim) ? base[rr + 1 + (int64_t)kk * ld] : 0u;
+          vr[p][e] = x0 | (x1 << 16);
         }
       }
     }
   }
 
-  // half e (0..3) of a 4-half word pair
-  __device__ __forceinline__ static uint32_t half_of(const u32x2& w, int e) {
-    return ((e < 2 ? w[0] : w[1]) >> (16 * (e & 1))) & 0xffffu;
+  // 8 halves of row r + h (h = 0: low halves, 1: high halves) of a !KCONT item
+  __device__ __forceinline__ u32x4 gather_row(int p, int h) const {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t a = (vr[p][2 * q] >> (16 * h)) & 0xffffu, b = (vr[p][2 * q + 1] >> (16 * h)) & 0xffffu;
+      w[q] = a | (b << 16);
+    }
+    return u32x4{w[0], w[1], w[2], w[3]};
   }
 
   __device__ __forceinline__ void store_raw(uint4* __restrict__ dst, int tid) const {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-      const int c = k >> 3;
+      int r, c;
+      item(tid + NT * p, r, c);
       if constexpr (KCONT) {
-        *reinterpret_cast<u32x4*>(&dst[lds_slot<R>(c, r)]) = vk[p];
+        *reinterpret_cast<u32x4*>(&dst[lds_slot_rm<KC>(r, c)]) = vk[p];
       } else {
-        const int kh = (k >> 2) & 1;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const u32x2 o{half_of(vr[p][0], e) | (half_of(vr[p][1], e) << 16),
-                        half_of(vr[p][2], e) | (half_of(vr[p][3], e) << 16)};
-          reinterpret_cast<u32x2*>(&dst[lds_slot<R>(c, r + e)])[kh] = o;
-        }
+        for (int h = 0; h < 2; ++h) *reinterpret_cast<u32x4*>(&dst[lds_slot_rm<KC>(r + h, c)]) = gather_row(p, h);
       }
+    }
+  }
+
+  __device__ __forceinline__ static void to_bf16x2(const u32x4& w, bf16x8& x0, bf16x8& x1) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint32_t hbits = (w[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+      __bf16 b0, b1;
+      split_bf16x2(bits_to_f16((uint16_t)hbits), b0, b1);
+      x0[e] = b0;
+      x1[e] = b1;
     }
   }
 
   __device__ __forceinline__ void store_bf16(uint4* __restrict__ d0, uint4* __restrict__ d1, int tid) const {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      int r, k;
-      item(tid + 256 * p, r, k);
-      const int c = k >> 3;
-      if constexpr (KCONT) {
-        bf16x8 x0, x1;
+      int r, c;
+      item(tid + NT * p, r, c);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const uint32_t h = (vk[p][e >> 1] >> (16 * (e & 1))) & 0xffffu;
-          __bf16 b0, b1;
-          split_bf16x2(bits_to_f16((uint16_t)h), b0, b1);
-          x0[e] = b0;
-          x1[e] = b1;
-        }
-        const int s = lds_slot<R>(c, r);
+      for (int h = 0; h < ROWS_PER_ITEM; ++h) {
+        bf16x8 x0, x1;
+        if constexpr (KCONT) to_bf16x2(vk[p], x0, x1);
+        else to_bf16x2(gather_row(p, h), x0, x1);
+        const int s = lds_slot_rm<KC>(r + h, c);
         *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
         *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
-      } else {
-        const int kh = (k >> 2) & 1;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          bf16x4 x0, x1;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            __bf16 b0, b1;
-            split_bf16x2(bits_to_f16((uint16_t)half_of(vr[p][j], e)), b0, b1);
-            x0[j] = b0;
-            x1[j] = b1;
-          }
-          const int s = lds_slot<R>(c, r + e);
-          reinterpret_cast<bf16x4*>(&d0[s])[kh] = x0;
-          reinterpret_cast<bf16x4*>(&d1[s])[kh] = x1;
-        }
       }
     }
   }
@@ -351,7 +291,9 @@ struct TileCfg {
   // (register sets); OCC: workgroups per CU the register budget targets.
   static constexpr int PF = PF_, OCC = OCC_;
   static constexpr int KC = BK / 8;
-  static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
+  static constexpr int WAVES = (BM / WM) * (BN / WN);
+  static constexpr int NT = 64 * WAVES;  // threads per workgroup
+  static_assert(WAVES == 4 || WAVES == 8 || WAVES == 16, "4, 8 or 16 wavefronts per workgroup");
   static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 16 == 0, "MFMA 32x32x16 granularity");
   static_assert(PF == 1 || PF == 2, "one or two k-tiles of prefetch");
   template <int MODE>
@@ -368,9 +310,11 @@ struct TileCfg {
 // include/shgemm/shgemm.hpp:30-34, plus the split-K tile):
 //   P1: large tiles; P0: small tiles for shapes with too few large tiles;
 //   SK: split-K (tall-skinny shapes have small m and n).
-// The prescaled mode has one accumulator, so its 64 x 128 wave tiles fit two
-// workgroups per CU; the two-accumulator modes use 64 x 64 wave tiles.
-// Chosen by the tile sweep at m = n = k = 8192 and 16384.
+// The prescaled mode has one accumulator: its large tile runs 8 wavefronts of
+// 64 x 64 with two workgroups per CU (4 wavefronts per SIMD to overlap the
+// split/store phase of one with the MFMA phase of another).  The
+// two-accumulator modes use 4 wavefronts of 64 x 64.  Chosen by the tile
+// sweep at m = n = k = 8192 and 16384.
 template <int MODE>
 struct ModeCfg {
   using P1 = TileCfg<128, 128, 32, 64, 64, 1, 2>;
@@ -379,7 +323,7 @@ struct ModeCfg {
 };
 template <>
 struct ModeCfg<MODE_FP16S> {
-  using P1 = TileCfg<128, 256, 32, 64, 128, 1, 2>;
+  using P1 = TileCfg<128, 256, 32, 64, 64, 1, 2>;  // 8 wavefronts, 4 per SIMD
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
 };
@@ -391,7 +335,7 @@ struct ModeCfg<MODE_FP16S> {
 //   SPLITK: blockIdx.y selects a k-slice; the partial goes to p.ws.
 // ---------------------------------------------------------------------------
 template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
-__global__ void __launch_bounds__(256, CFG::OCC) shgemm_kernel(const GemmParams p) {
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmParams p) {
   constexpr bool SCALED = (MODE == MODE_FP16S);
   constexpr int NACC = (MODE == MODE_FP16) ? 2 : 1;  // separate hi / lo accumulators
   constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC;
@@ -407,8 +351,9 @@ __global__ void __launch_bounds__(256, CFG::OCC) shgemm_kernel(const GemmParams 
   constexpr int PF = CFG::PF;
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, PF - 1>;
-  TileF32<OPA == OP_T, BM, BK> ta[PF];
-  TileF16<OPB == OP_N, BN, BK> tb[PF];
+  constexpr int NT = CFG::NT;
+  TileF32<OPA == OP_T, BM, BK, NT> ta[PF];
+  TileF16<OPB == OP_N, BN, BK, NT> tb[PF];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
@@ -472,13 +417,13 @@ __global__ void __launch_bounds__(256, CFG::OCC) shgemm_kernel(const GemmParams 
         f16x8 ah[FM], al[FM], bb[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          const int sl = lds_slot<BM>(c, wm * WM + i * 32 + lrow);
+          const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
           ah[i] = *reinterpret_cast<const f16x8*>(&st[sl]);
           al[i] = *reinterpret_cast<const f16x8*>(&st[A_CH + sl]);
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          bb[j] = *reinterpret_cast<const f16x8*>(&st[NA * A_CH + lds_slot<BN>(c, wn * WN + j * 32 + lrow)]);
+          bb[j] = *reinterpret_cast<const f16x8*>(&st[NA * A_CH + lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -495,14 +440,14 @@ __global__ void __launch_bounds__(256, CFG::OCC) shgemm_kernel(const GemmParams 
         bf16x8 a0[FM], a1[FM], a2[FM], b0[FN], b1[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          const int sl = lds_slot<BM>(c, wm * WM + i * 32 + lrow);
+          const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
           a0[i] = *reinterpret_cast<const bf16x8*>(&st[sl]);
           a1[i] = *reinterpret_cast<const bf16x8*>(&st[A_CH + sl]);
           a2[i] = *reinterpret_cast<const bf16x8*>(&st[2 * A_CH + sl]);
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          const int sl = lds_slot<BN>(c, wn * WN + j * 32 + lrow);
+          const int sl = lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c);
           b0[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + sl]);
           b1[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + B_CH + sl]);
         }

@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--sample-rows", type=int, default=256, help="rows of C checked against FP64")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device copies (reference for FETCH_SIZE/WRITE_SIZE scaling)")
     return ap.parse_args()
 
 
@@ -109,6 +111,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+        return
 
     import shgemm_amd as sg
 

@@ -108,6 +108,16 @@ __device__ __forceinline__ void split_bf16x2(_Float16 b, __bf16& b0, __bf16& b1)
   b1 = (__bf16)(f - (float)b0);
 }
 
+// A pointer the compiler must treat as wave-uniform (held in scalar
+// registers), so loads off it use the scalar-base + 32-bit-offset form.
+template <class T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ _Float16 bits_to_f16(uint16_t u) {
   return __builtin_bit_cast(_Float16, u);
 }

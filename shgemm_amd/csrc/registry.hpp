@@ -22,6 +22,8 @@ struct KernelTable {
   KernelEntry shgemm[2][2][2][kNumModes][2][2];
   // [op_a][op_b][op_c]
   KernelEntry sgemm[2][2][2];
+  // deep-prefetch prescaled fp16 kernel (op_b = n, full k-tiles): [op_a][op_c]
+  KernelEntry shgemm_bd[2][2];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,
   // timed by the tile sweep and selectable with MTK_SHGEMM_TUNE_P1=<index>.
   static constexpr int kMaxTune = 16;
@@ -43,6 +45,7 @@ void register_shgemm_fp16s_tn(KernelTable&);
 void register_shgemm_fp16s_tt(KernelTable&);
 void register_sgemm(KernelTable&);
 void register_tune(KernelTable&);
+void register_shgemm_bd(KernelTable&);
 void register_all(KernelTable&);
 
 // Auxiliary kernels (defined in aux_kernels.hip).

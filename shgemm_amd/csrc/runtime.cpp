@@ -30,7 +30,8 @@ struct handle_state {
   int* row_exp = nullptr;
   size_t row_cap = 0;
   bool exp_scaling = true;
-  int tune_p1 = -1;  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
+  int tune_p1 = -1;
+  bool use_bd = false;  // deep-prefetch kernel where it applies (opt-in: MTK_SHGEMM_BD=1)  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
 };
 
 }  // namespace detail
@@ -132,6 +133,13 @@ void create(shgemmHandle_t& handle) {
     if (i < st->table.ntune) st->tune_p1 = i;
   }
   for (int i = 0; i < st->table.ntune; ++i) set_lds_limit(st->table.tune[i]);
+  for (auto& a : st->table.shgemm_bd)
+    for (auto& e : a)
+      if (e.fn) set_lds_limit(e);
+  // Opt-in: at 176 VGPRs it runs 2 wavefronts per SIMD and measured slower
+  // than the default large tile at 16384^3 (389 vs 473 TFLOP/s effective).
+  const char* bd = std::getenv("MTK_SHGEMM_BD");
+  st->use_bd = (bd != nullptr && bd[0] == '1');
   // Per-row exponent prescaling of the fp16 compute type is on unless
   // MTK_SHGEMM_EXP_SCALING=0 (then fp16 has FP16's exponent range, as in the reference).
   const char* sc = std::getenv("MTK_SHGEMM_EXP_SCALING");
@@ -279,7 +287,13 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   }
   const bool tuned = st->tune_p1 >= 0 && level == detail::P1 && st->table.tune[st->tune_p1].mode == mode &&
                      oa == shgemm_amd::OP_N && ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N;
-  const auto& e = tuned ? st->table.tune[st->tune_p1] : tab[level][0];
+  // Deep-prefetch kernel: prescaled fp16, op_b = n, large tiles, full k-tiles,
+  // B rows 16-byte aligned; A needs 16-byte alignment only when k-contiguous.
+  const auto& bd = st->table.shgemm_bd[oa][oc];
+  const bool use_bd = !tuned && st->use_bd && bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S &&
+                      level == detail::P1 && ob == shgemm_amd::OP_N && p.vec_b && (k % bd.bk) == 0 &&
+                      (oa == shgemm_amd::OP_N || p.vec_a);
+  const auto& e = tuned ? st->table.tune[st->tune_p1] : use_bd ? bd : tab[level][0];
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
   p.group_m = group_rows(p.tiles_m);
@@ -356,5 +370,6 @@ void register_all(KernelTable& t) {
   register_shgemm_fp16s_tt(t);
   register_sgemm(t);
   register_tune(t);
+  register_shgemm_bd(t);
 }
 }  // namespace shgemm_amd

@@ -94,6 +94,47 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
     }
   }
 
+  // Full-k-tile load with rows past rlim clamped to the last row: the values
+  // only reach output rows that are never stored, and every access stays in
+  // the operand.  Always issues the same instructions (PER*2 or PER*8 loads).
+  // Addresses are a wave-uniform base (scalar registers) plus a 32-bit
+  // per-lane offset, so the loads use the scalar-base addressing form; the
+  // caller guarantees the tile's offsets fit in 32 bits.
+  __device__ __forceinline__ void load_clamped(const float* __restrict__ base, int64_t ld, int r0, int k0, int rlim,
+                                               int tid) {
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      if constexpr (KCONT) {
+        int r, c;
+        M::item(tid + NT * p, r, c);
+        const int rr = min(r0 + r, rlim - 1) - r0;
+        const char* tile = uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)r0 * ld + k0));
+        const uint32_t boff = 4u * ((uint32_t)rr * (uint32_t)ld + 8u * (uint32_t)c);
+        const f32x4 x = *reinterpret_cast<const f32x4*>(tile + boff);
+        const f32x4 y = *reinterpret_cast<const f32x4*>(tile + boff + 16);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[p][e] = x[e];
+          v[p][4 + e] = y[e];
+        }
+      } else {
+        // item = 64 * wblk + lane with wblk wave-uniform: r = 64*(wblk % (R/64)) + lane, c = wblk / (R/64)
+        const int wblk = wave + (NT / 64) * p;
+        const int r = 64 * (wblk % (R / 64)) + lane;
+        const int c = wblk / (R / 64);
+        const uint32_t rbytes = 4u * (uint32_t)min(r0 + r, rlim - 1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const char* col = uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)(k0 + 8 * c + e) * ld));
+          v[p][e] = *reinterpret_cast<const float*>(col + rbytes);
+        }
+      }
+    }
+  }
+  static constexpr int kClampedLoads = KCONT ? 2 * PER : 8 * PER;
+
   __device__ __forceinline__ void load_safe(const float* __restrict__ base, int64_t ld, int r0, int k0,
                                             int rlim, int klim, int tid) {
 #pragma unroll
@@ -282,13 +323,54 @@ struct TileF16 {
 };
 
 // ---------------------------------------------------------------------------
+// k-contiguous FP16 operand loaded straight into LDS (global_load_lds_dwordx4:
+// lane l of an instruction writes 16 bytes at the instruction's LDS base +
+// 16*l, measured on gfx950).  Instruction q of wavefront w fills physical
+// slots [(q*WAVES + w)*64, +64); the lane at physical slot (r, cp) fetches
+// logical chunk cp ^ key(r), the inverse of lds_slot_rm, so the LDS image
+// equals what TileF16::store_raw writes.  Rows past rlim are clamped (their
+// values only reach columns that are never stored); the caller guarantees a
+// full k-tile, 16-byte-aligned rows and ld % 8 == 0.
+// ---------------------------------------------------------------------------
+template <int R, int BK, int NT>
+struct DirectF16 {
+  static constexpr int KC = BK / 8;
+  static constexpr int WAVES = NT / 64;
+  static constexpr int PER = R * KC / NT;  // instructions per thread per k-tile
+  static_assert((R * KC) % NT == 0 && PER >= 1, "tile must split into whole wavefront instructions");
+  __device__ __forceinline__ static void load(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0, int rlim,
+                                              uint4* __restrict__ dst, int tid) {
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const char* tile = reinterpret_cast<const char*>(base + (int64_t)r0 * ld + k0);  // uniform; offsets fit 32 bits
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int blk = q * WAVES + wave;
+      const int slot = blk * 64 + lane;
+      const int r = slot / KC, cp = slot % KC;
+      const int c = cp ^ ((r / (16 / KC)) & (KC - 1));
+      const uint32_t rr = (uint32_t)(min(r0 + r, rlim - 1) - r0);
+      const uint32_t boff = 2u * (rr * (uint32_t)ld + 8u * (uint32_t)c);
+      __builtin_amdgcn_global_load_lds(tile + boff, reinterpret_cast<uint32_t*>(dst + blk * 64), 16, 0, 0);
+    }
+  }
+};
+
+// s_waitcnt immediate for "at most n vector-memory operations outstanding",
+// other counters left unconstrained (gfx9 encoding).
+__device__ __forceinline__ constexpr int vmcnt_imm(int n) {
+  return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
+}
+
+// ---------------------------------------------------------------------------
 // Tile configuration
 // ---------------------------------------------------------------------------
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int PF_, int OCC_>
 struct TileCfg {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   // PF: k-tiles of global loads in flight ahead of the one being multiplied
-  // (register sets); OCC: workgroups per CU the register budget targets.
+  // (register sets); OCC: minimum wavefronts per SIMD the register budget
+  // must allow (the second __launch_bounds__ argument in HIP).
   static constexpr int PF = PF_, OCC = OCC_;
   static constexpr int KC = BK / 8;
   static constexpr int WAVES = (BM / WM) * (BN / WN);
@@ -327,6 +409,120 @@ struct ModeCfg<MODE_FP16S> {
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
 };
+
+// One BK-deep k-tile of MFMA work for one wavefront: fragments from the A
+// parts at ast (+ q*BM*KC slots per part) and the B parts at bst (+ q*BN*KC).
+template <int MODE, bool LANE_M, int NACC, int FM, int FN, int KC, int BM, int BN, int WM, int WN>
+__device__ __forceinline__ void mma_ktile(f32x16 (&acc)[NACC][FM][FN], const uint4* __restrict__ ast,
+                                          const uint4* __restrict__ bst, int wm, int wn, int lrow, int lh) {
+  constexpr int A_CH = BM * KC, B_CH = BN * KC, BK = KC * 8;
+#pragma unroll
+  for (int s = 0; s < BK / 16; ++s) {
+    const int c = 2 * s + lh;
+    if constexpr (MODE != MODE_BF16) {
+      f16x8 ah[FM], al[FM], bb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
+        ah[i] = *reinterpret_cast<const f16x8*>(&ast[sl]);
+        al[i] = *reinterpret_cast<const f16x8*>(&ast[A_CH + sl]);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if constexpr (LANE_M) {
+            acc[0][i][j] = mfma_f16(bb[j], ah[i], acc[0][i][j]);
+            acc[NACC - 1][i][j] = mfma_f16(bb[j], al[i], acc[NACC - 1][i][j]);
+          } else {
+            acc[0][i][j] = mfma_f16(ah[i], bb[j], acc[0][i][j]);
+            acc[NACC - 1][i][j] = mfma_f16(al[i], bb[j], acc[NACC - 1][i][j]);
+          }
+        }
+    } else {
+      bf16x8 a0[FM], a1[FM], a2[FM], b0[FN], b1[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
+        a0[i] = *reinterpret_cast<const bf16x8*>(&ast[sl]);
+        a1[i] = *reinterpret_cast<const bf16x8*>(&ast[A_CH + sl]);
+        a2[i] = *reinterpret_cast<const bf16x8*>(&ast[2 * A_CH + sl]);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int sl = lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c);
+        b0[j] = *reinterpret_cast<const bf16x8*>(&bst[sl]);
+        b1[j] = *reinterpret_cast<const bf16x8*>(&bst[B_CH + sl]);
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          // a0*b0 + a0*b1 + a1*b0 + a1*b1 + a2*b0 (a2*b1 ~ 2^-24 dropped), all in
+          // one accumulator: each MFMA adds its exact product sum with one
+          // round-to-nearest-even.
+          f32x16& c = acc[0][i][j];
+          if constexpr (LANE_M) {
+            c = mfma_bf16(b0[j], a0[i], c);
+            c = mfma_bf16(b1[j], a0[i], c);
+            c = mfma_bf16(b0[j], a1[i], c);
+            c = mfma_bf16(b1[j], a1[i], c);
+            c = mfma_bf16(b0[j], a2[i], c);
+          } else {
+            c = mfma_bf16(a0[i], b0[j], c);
+            c = mfma_bf16(a0[i], b1[j], c);
+            c = mfma_bf16(a1[i], b0[j], c);
+            c = mfma_bf16(a1[i], b1[j], c);
+            c = mfma_bf16(a2[i], b0[j], c);
+          }
+        }
+    }
+  }
+}
+
+// Epilogue.  Lane/register -> (m, n) of the 32x32 MFMA output block: the lane
+// index gives the column, register e gives row (e/4)*8 + (lane/32)*4 + e%4.
+template <int MODE, bool LANE_M, bool SPLITK, int NACC, int FM, int FN, int WM, int WN>
+__device__ __forceinline__ void store_ctile(const f32x16 (&acc)[NACC][FM][FN], const GemmParams& p, int m0, int n0,
+                                            int wm, int wn, int lrow, int lh) {
+  constexpr bool SCALED = (MODE == MODE_FP16S);
+  constexpr float lo_w = 1.0f / 2048.0f;
+  const float alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int ridx = (e >> 2) * 8 + lh * 4 + (e & 3);
+        int mm, nn;
+        if constexpr (LANE_M) {
+          mm = m0 + wm * WM + i * 32 + lrow;
+          nn = n0 + wn * WN + j * 32 + ridx;
+        } else {
+          mm = m0 + wm * WM + i * 32 + ridx;
+          nn = n0 + wn * WN + j * 32 + lrow;
+        }
+        if (mm < p.m && nn < p.n) {
+          float v = acc[0][i][j][e];
+          if constexpr (NACC == 2) v = __builtin_fmaf(acc[1][i][j][e], lo_w, v);
+          if constexpr (SCALED) v = __builtin_ldexpf(v, p.row_exp[mm]);
+          if constexpr (SPLITK) {
+            const int64_t off = (int64_t)blockIdx.y * p.m * p.n +
+                                (LANE_M ? (mm + (int64_t)nn * p.m) : (nn + (int64_t)mm * p.n));
+            p.ws[off] = v;
+          } else {
+            float* cp = p.c + (LANE_M ? (mm + nn * p.ldc) : (nn + mm * p.ldc));
+            *cp = (beta == 0.0f) ? alpha * v : alpha * v + beta * *cp;
+          }
+        }
+      }
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // The kernel.
@@ -410,71 +606,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
     }
   };
   auto compute = [&](const uint4* __restrict__ st) {
-#pragma unroll
-    for (int s = 0; s < BK / 16; ++s) {
-      const int c = 2 * s + lh;
-      if constexpr (MODE != MODE_BF16) {
-        f16x8 ah[FM], al[FM], bb[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
-          ah[i] = *reinterpret_cast<const f16x8*>(&st[sl]);
-          al[i] = *reinterpret_cast<const f16x8*>(&st[A_CH + sl]);
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-          bb[j] = *reinterpret_cast<const f16x8*>(&st[NA * A_CH + lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            if constexpr (LANE_M) {
-              acc[0][i][j] = mfma_f16(bb[j], ah[i], acc[0][i][j]);
-              acc[NACC - 1][i][j] = mfma_f16(bb[j], al[i], acc[NACC - 1][i][j]);
-            } else {
-              acc[0][i][j] = mfma_f16(ah[i], bb[j], acc[0][i][j]);
-              acc[NACC - 1][i][j] = mfma_f16(al[i], bb[j], acc[NACC - 1][i][j]);
-            }
-          }
-      } else {
-        bf16x8 a0[FM], a1[FM], a2[FM], b0[FN], b1[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
-          a0[i] = *reinterpret_cast<const bf16x8*>(&st[sl]);
-          a1[i] = *reinterpret_cast<const bf16x8*>(&st[A_CH + sl]);
-          a2[i] = *reinterpret_cast<const bf16x8*>(&st[2 * A_CH + sl]);
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          const int sl = lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c);
-          b0[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + sl]);
-          b1[j] = *reinterpret_cast<const bf16x8*>(&st[NA * A_CH + B_CH + sl]);
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            // a0*b0 + a0*b1 + a1*b0 + a1*b1 + a2*b0 (a2*b1 ~ 2^-24 dropped), all in
-            // one accumulator: each MFMA adds its exact product sum with one
-            // round-to-nearest-even.
-            f32x16& c = acc[0][i][j];
-            if constexpr (LANE_M) {
-              c = mfma_bf16(b0[j], a0[i], c);
-              c = mfma_bf16(b1[j], a0[i], c);
-              c = mfma_bf16(b0[j], a1[i], c);
-              c = mfma_bf16(b1[j], a1[i], c);
-              c = mfma_bf16(b0[j], a2[i], c);
-            } else {
-              c = mfma_bf16(a0[i], b0[j], c);
-              c = mfma_bf16(a0[i], b1[j], c);
-              c = mfma_bf16(a1[i], b0[j], c);
-              c = mfma_bf16(a1[i], b1[j], c);
-              c = mfma_bf16(a2[i], b0[j], c);
-            }
-          }
-      }
-    }
+    mma_ktile<MODE, LANE_M, NACC, FM, FN, KC, BM, BN, WM, WN>(acc, st, st + NA * A_CH, wm, wn, lrow, lh);
   };
 
   // LDS double buffer, one barrier per k-tile.  The store into stage cur^1 is
@@ -514,41 +646,113 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
     }
   }
 
-  // Epilogue.  Lane/register -> (m, n) of the 32x32 MFMA output block:
-  // lane index gives the column, register j gives row (j/4)*8 + (lane/32)*4 + j%4.
-  constexpr float lo_w = 1.0f / 2048.0f;
-  const float alpha = p.alpha, beta = p.beta;
+  store_ctile<MODE, LANE_M, SPLITK, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh);
+}
+
+// ---------------------------------------------------------------------------
+// Deep-prefetch variant of the prescaled fp16 kernel for op_b = op_n with
+// full k-tiles (k % BK == 0) and aligned B: B goes straight to LDS into a
+// 3-stage ring and A through two register sets, both issued two k-tiles
+// ahead of the MFMA work, so each load has two k-tiles of MFMA time to land.
+// Every thread issues the same number of vector-memory instructions per
+// k-tile (no bounds-checked paths: edge rows are clamped), which lets the
+// barrier wait with an exact vmcnt for the tile it publishes.
+// LDS: A hi/lo x 2 stages, B x 3 stages.
+// ---------------------------------------------------------------------------
+template <class CFG>
+struct BdLayout {
+  static constexpr int KC = CFG::KC;
+  static constexpr int A_STAGE = 2 * CFG::BM * KC;  // hi + lo slots
+  static constexpr int B_STAGE = CFG::BN * KC;
+  static constexpr size_t lds_bytes() { return (size_t)(2 * A_STAGE + 3 * B_STAGE) * 16; }
+};
+
+template <int OPA, int OPC, class CFG>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_bd(const GemmParams p) {
+  constexpr int MODE = MODE_FP16S;
+  constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM, NT = CFG::NT;
+  constexpr int A_CH = BM * KC;
+  constexpr int A_STAGE = BdLayout<CFG>::A_STAGE, B_STAGE = BdLayout<CFG>::B_STAGE;
+  constexpr bool LANE_M = (OPC == OP_N);
+  using TA = TileF32<OPA == OP_T, BM, BK, NT>;
+  using DB = DirectF16<BN, BK, NT>;
+  // vector-memory instructions one k-tile issues per thread
+  constexpr int VM_PER_TILE = TA::kClampedLoads + DB::PER;
+  extern __shared__ uint4 lds[];
+  uint4* const a_lds = lds;                  // 2 stages
+  uint4* const b_lds = lds + 2 * A_STAGE;    // 3 stages
+
+  TA ta0, ta1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int lrow = lane & 31, lh = lane >> 5;
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int ktiles = p.k / BK;
+  const float* __restrict__ A = p.a;
+  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+  ta0.init_scale(p.row_exp, m0, p.m, tid);
+  ta1.init_scale(p.row_exp, m0, p.m, tid);
+
+  f32x16 acc[1][FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int ridx = (e >> 2) * 8 + lh * 4 + (e & 3);
-        int mm, nn;
-        if constexpr (LANE_M) {
-          mm = m0 + wm * WM + i * 32 + lrow;
-          nn = n0 + wn * WN + j * 32 + ridx;
-        } else {
-          mm = m0 + wm * WM + i * 32 + ridx;
-          nn = n0 + wn * WN + j * 32 + lrow;
-        }
-        if (mm < p.m && nn < p.n) {
-          float v = acc[0][i][j][e];
-          if constexpr (NACC == 2) v = __builtin_fmaf(acc[1][i][j][e], lo_w, v);
-          if constexpr (SCALED) v = __builtin_ldexpf(v, p.row_exp[mm]);
-          if constexpr (SPLITK) {
-            const int64_t off = (int64_t)blockIdx.y * p.m * p.n +
-                                (LANE_M ? (mm + (int64_t)nn * p.m) : (nn + (int64_t)mm * p.n));
-            p.ws[off] = v;
-          } else {
-            float* cp = p.c + (LANE_M ? (mm + nn * p.ldc) : (nn + mm * p.ldc));
-            *cp = (beta == 0.0f) ? alpha * v : alpha * v + beta * *cp;
-          }
-        }
-      }
-    }
+      for (int e = 0; e < 16; ++e) acc[0][i][j][e] = 0.0f;
+
+  // issue one k-tile: B into ring slot kt%3, A into the given register set
+  auto issue = [&](TA& ta, int kt) {
+    DB::load(B, p.ldb, n0, kt * BK, p.n, b_lds + (kt % 3) * B_STAGE, tid);
+    ta.load_clamped(A, p.lda, m0, kt * BK, p.m, tid);
+  };
+  auto store_a = [&](const TA& ta, int kt) {
+    uint4* st = a_lds + (kt & 1) * A_STAGE;
+    ta.template store_f16<true>(st, st + A_CH, tid);
+  };
+  auto compute = [&](int kt) {
+    mma_ktile<MODE, LANE_M, 1, FM, FN, KC, BM, BN, WM, WN>(acc, a_lds + (kt & 1) * A_STAGE,
+                                                          b_lds + (kt % 3) * B_STAGE, wm, wn, lrow, lh);
+  };
+  // Publish k-tile kt+1 to LDS: its A registers are stored by this thread, and
+  // its B load must have landed; at most one newer k-tile of loads may still
+  // be in flight (`newer`).
+  auto publish = [&](bool newer) {
+    if (newer) __builtin_amdgcn_s_waitcnt(vmcnt_imm(VM_PER_TILE));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    __syncthreads();
+  };
+
+  if (ktiles > 0) {
+    issue(ta0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (ktiles > 1) issue(ta1, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store_a(ta0, 0);
+    publish(ktiles > 1);
   }
+  for (int kt = 0; kt < ktiles; kt += 2) {
+    // even step: ta0 is free (tile kt stored), ta1 holds tile kt+1
+    if (kt + 2 < ktiles) issue(ta0, kt + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(kt);
+    __builtin_amdgcn_sched_barrier(0);
+    if (kt + 1 < ktiles) store_a(ta1, kt + 1);
+    publish(kt + 2 < ktiles);
+    if (kt + 1 >= ktiles) break;
+    // odd step: ta1 is free, ta0 holds tile kt+2
+    if (kt + 3 < ktiles) issue(ta1, kt + 3);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(kt + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    if (kt + 2 < ktiles) store_a(ta0, kt + 2);
+    publish(kt + 3 < ktiles);
+  }
+
+  store_ctile<MODE, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh);
 }
 
 }  // namespace shgemm_amd

@@ -43,37 +43,77 @@ __global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t* __restrict__ p,
   if (i < n) p[i] = v;
 }
 
+__device__ __forceinline__ uint32_t abs_bits(float x) { return __builtin_bit_cast(uint32_t, x) & 0x7fffffffu; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
 // A stored with rows contiguous (op_n): element (r, k) at a[r + k*lda].
-// grid.x covers rows in 256s, grid.y covers k in chunks of kchunk.
+// A workgroup covers `rq` row-quads (4 consecutive rows each) and 256/rq
+// k-lanes, so narrow matrices (small m) still keep every thread loading.
+// With `vec` a row-quad at one k is one 16-byte load (rows 4-aligned,
+// lda % 4 == 0, 16-byte base).  grid.x covers row-quads in groups of rq,
+// grid.y covers k in chunks of kchunk; the max is order-independent.
 __global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                              int kchunk, uint32_t* __restrict__ row_max) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= m) return;
-  const int k0 = blockIdx.y * kchunk, k1 = min(k, k0 + kchunk);
-  uint32_t mx = 0;
-  for (int kk = k0; kk < k1; ++kk) {
-    const uint32_t b = __builtin_bit_cast(uint32_t, a[r + (int64_t)kk * lda]) & 0x7fffffffu;
-    mx = b > mx ? b : mx;
+                                                              int kchunk, int rq, int vec,
+                                                              uint32_t* __restrict__ row_max) {
+  const int q = threadIdx.x % rq, kl = threadIdx.x / rq, klanes = 256 / rq;
+  const int r4 = (blockIdx.x * rq + q) * 4;
+  uint32_t mx[4] = {0u, 0u, 0u, 0u};
+  if (r4 < m && kl < klanes) {
+    const int k0 = blockIdx.y * kchunk, k1 = min(k, k0 + kchunk);
+    if (vec && r4 + 4 <= m) {
+#pragma unroll 4
+      for (int kk = k0 + kl; kk < k1; kk += klanes) {
+        const float4 v = *reinterpret_cast<const float4*>(a + r4 + (int64_t)kk * lda);
+        mx[0] = umax(mx[0], abs_bits(v.x));
+        mx[1] = umax(mx[1], abs_bits(v.y));
+        mx[2] = umax(mx[2], abs_bits(v.z));
+        mx[3] = umax(mx[3], abs_bits(v.w));
+      }
+    } else {
+      for (int kk = k0 + kl; kk < k1; kk += klanes)
+        for (int e = 0; e < 4; ++e)
+          if (r4 + e < m) mx[e] = umax(mx[e], abs_bits(a[r4 + e + (int64_t)kk * lda]));
+    }
   }
-  atomicMax(&row_max[r], mx);
+  // combine the k-lanes of each row-quad in LDS, then one atomic per row
+  __shared__ uint32_t red[4][256];
+  for (int e = 0; e < 4; ++e) red[e][threadIdx.x] = mx[e];
+  __syncthreads();
+  if (kl == 0 && r4 < m) {
+    for (int e = 0; e < 4; ++e) {
+      uint32_t v = 0;
+      for (int j = 0; j < klanes; ++j) v = umax(v, red[e][q + j * rq]);
+      if (r4 + e < m) atomicMax(&row_max[r4 + e], v);
+    }
+  }
 }
 
 // A stored with k contiguous (op_t): element (r, k) at a[r*lda + k].
-// One workgroup per (row, k chunk); a block reduction then one atomic.
+// One workgroup per (row, k chunk): 16-byte loads when `vec`, a block
+// reduction, then one atomic.
 __global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                              int kchunk, uint32_t* __restrict__ row_max) {
+                                                              int kchunk, int vec, uint32_t* __restrict__ row_max) {
   const int r = blockIdx.x;
   const int k0 = blockIdx.y * kchunk, k1 = min(k, k0 + kchunk);
+  const float* row = a + (int64_t)r * lda;
   uint32_t mx = 0;
-  for (int kk = k0 + threadIdx.x; kk < k1; kk += 256) {
-    const uint32_t b = __builtin_bit_cast(uint32_t, a[(int64_t)r * lda + kk]) & 0x7fffffffu;
-    mx = b > mx ? b : mx;
+  if (vec) {
+    const int v0 = (k0 + 3) / 4 * 4, v1 = k1 / 4 * 4;  // 16-byte aligned middle
+    for (int kk = k0 + threadIdx.x; kk < min(v0, k1); kk += 256) mx = umax(mx, abs_bits(row[kk]));
+#pragma unroll 4
+    for (int kk = v0 + 4 * threadIdx.x; kk < v1; kk += 1024) {
+      const float4 v = *reinterpret_cast<const float4*>(row + kk);
+      mx = umax(umax(mx, abs_bits(v.x)), umax(abs_bits(v.y), umax(abs_bits(v.z), abs_bits(v.w))));
+    }
+    for (int kk = max(v1, v0) + threadIdx.x; kk < k1; kk += 256) mx = umax(mx, abs_bits(row[kk]));
+  } else {
+    for (int kk = k0 + threadIdx.x; kk < k1; kk += 256) mx = umax(mx, abs_bits(row[kk]));
   }
   __shared__ uint32_t red[256];
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    if (threadIdx.x < s) red[threadIdx.x] = umax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicMax(&row_max[r], red[0]);
@@ -114,15 +154,25 @@ hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, in
   hipLaunchKernelGGL(fill_u32_kernel, dim3(rb), dim3(256), 0, s, row_max, m, 0u);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
+  const bool aligned16 = (reinterpret_cast<uintptr_t>(a) % 16) == 0 && (lda % 4) == 0;
   if (k > 0) {
     if (op_a == OP_N) {
-      const int kchunk = std::max(256, (int)((k + 65534) / 65535));
+      // rq row-quads per workgroup (a power of two up to 256), the rest of the
+      // 256 threads spread over k; about 64 loads per thread; grid.y <= 65535
+      const int quads = (m + 3) / 4;
+      int rq = 1;
+      while (rq < 256 && rq < quads) rq *= 2;
+      const int klanes = 256 / rq;
+      const int kchunk = std::max(64 * klanes, (int)((k + 65534) / 65535));
       const unsigned kb = (unsigned)((k + kchunk - 1) / kchunk);
-      hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3(rb, kb), dim3(256), 0, s, a, lda, m, k, kchunk, row_max);
+      const unsigned rbq = (unsigned)((quads + rq - 1) / rq);
+      hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3(rbq, kb), dim3(256), 0, s, a, lda, m, k, kchunk, rq,
+                         aligned16 ? 1 : 0, row_max);
     } else {
       const int kchunk = std::max(4096, (int)((k + 65534) / 65535));
       const unsigned kb = (unsigned)((k + kchunk - 1) / kchunk);
-      hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)m, kb), dim3(256), 0, s, a, lda, m, k, kchunk, row_max);
+      hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)m, kb), dim3(256), 0, s, a, lda, m, k, kchunk,
+                         aligned16 ? 1 : 0, row_max);
     }
     err = hipGetLastError();
     if (err != hipSuccess) return err;

@@ -8,8 +8,8 @@
 namespace shgemm_amd {
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
-inline void put_shgemm(KernelTable& t, int level) {
-  KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][level][SPLITK ? 1 : 0];
+inline void put_shgemm(KernelTable& t, int variant) {
+  KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][variant];
   e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OPA, OPB, OPC, MODE, CFG, SPLITK>);
   e.lds = CFG::template lds_bytes<MODE>();
   e.bm = CFG::BM;
@@ -22,9 +22,11 @@ inline void put_shgemm(KernelTable& t, int level) {
 template <int OPA, int OPB, int OPC, int MODE>
 inline void put_shgemm_opc(KernelTable& t) {
   using C = ModeCfg<MODE>;
-  put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, false>(t, LEVEL_P1);
-  put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, false>(t, LEVEL_P0);
-  put_shgemm<OPA, OPB, OPC, MODE, typename C::SK, true>(t, LEVEL_P1);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, false>(t, V_P1);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, false>(t, V_P0);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::SK, true>(t, V_SK);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, true>(t, V_P1SK);
+  put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, true>(t, V_P0SK);
 }
 
 template <int OPA, int OPB, int MODE>

@@ -17,9 +17,21 @@ struct KernelEntry {
 
 enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };
 
+// Kernel variants per (op_a, op_b, op_c, mode): tile config x split-K.
+enum : int {
+  V_P0 = 0,    // small tile
+  V_P1 = 1,    // large tile
+  V_SK = 2,    // 128 x 128 tile, split-K
+  V_P1SK = 3,  // large tile, split-K
+  V_P0SK = 4,  // small tile, split-K
+  kNumVariants = 5
+};
+inline bool variant_is_split(int v) { return v == V_SK || v == V_P1SK || v == V_P0SK; }
+inline int variant_level(int v) { return (v == V_P0 || v == V_P0SK) ? LEVEL_P0 : LEVEL_P1; }
+
 struct KernelTable {
-  // [op_a][op_b][op_c][mode][level][splitk]
-  KernelEntry shgemm[2][2][2][kNumModes][2][2];
+  // [op_a][op_b][op_c][mode][variant]
+  KernelEntry shgemm[2][2][2][kNumModes][kNumVariants];
   // [op_a][op_b][op_c]
   KernelEntry sgemm[2][2][2];
   // deep-prefetch prescaled fp16 kernel (op_b = n, full k-tiles): [op_a][op_c]

@@ -31,7 +31,9 @@ struct handle_state {
   size_t row_cap = 0;
   bool exp_scaling = true;
   int tune_p1 = -1;
-  bool use_bd = false;  // deep-prefetch kernel where it applies (opt-in: MTK_SHGEMM_BD=1)  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
+  bool use_bd = false;  // deep-prefetch kernel where it applies (opt-in: MTK_SHGEMM_BD=1)
+  int forced_variant = -1;  // MTK_SHGEMM_PLAN=<variant>,<slices>
+  int forced_slices = 1;  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
 };
 
 }  // namespace detail
@@ -95,6 +97,104 @@ void validate_common(const shgemmHandle_t& handle, operation_t op_a, operation_t
   }
 }
 
+// A kernel choice for one call: table variant (or an explicit entry) and the
+// number of k-slices.  `level` is what shgemm returns (reference
+// shgemm.cu:609, 683: P0 for the k-sliced path).
+struct Plan {
+  int variant = shgemm_amd::V_P1;
+  int slices = 1;
+  int level = detail::P1;
+  const shgemm_amd::KernelEntry* entry = nullptr;
+  const char* source = "";
+};
+
+// Largest useful split: every slice at least 8 k-tiles, all partials inside
+// the workspace.
+int64_t max_slices(const shgemm_amd::KernelEntry& e, int64_t m, int64_t n, int64_t k, size_t ws_bytes) {
+  int64_t s = 1;
+  while (k / (s * 2) >= 8 * e.bk && (s * 2) * m * n * (int64_t)sizeof(float) <= (int64_t)ws_bytes && s < 256) s *= 2;
+  return s;
+}
+
+// Estimated time (seconds) of one candidate on this device: the larger of an
+// MFMA-rate term (workgroup waves x per-workgroup work at the rate measured
+// for the tile) and an HBM term (operands once + C + split-K partials), plus
+// a fixed cost per extra launch.
+double plan_cost(const shgemm_amd::KernelEntry& e, int variant, int mode, int64_t slices, int64_t m, int64_t n,
+                 int64_t k, int ncu) {
+  const int64_t tiles = ((m + e.bm - 1) / e.bm) * ((n + e.bn - 1) / e.bn);
+  const int level = shgemm_amd::variant_level(variant);
+  // workgroups resident per CU and the effective MFMA rate one CU sustains on
+  // this tile (TFLOP/s effective per CU), from the tile sweep
+  const double per_cu_tflops = (mode == shgemm_amd::MODE_BF16 ? 0.8 : 1.8) * (level == 1 ? 1.0 : 0.55);
+  const int wg_per_cu = (e.threads >= 512) ? 2 : (level == 1 ? 2 : 4);
+  const int64_t wgs = tiles * slices;
+  const int64_t waves = (wgs + (int64_t)ncu * wg_per_cu - 1) / ((int64_t)ncu * wg_per_cu);
+  const double wg_flops = 2.0 * e.bm * e.bn * ((double)k / slices);
+  const double t_mfma = waves * wg_flops / (per_cu_tflops * 1e12 / wg_per_cu);
+  const double bytes = 4.0 * m * k + 2.0 * k * n + 4.0 * m * n + (slices > 1 ? 8.0 * slices * m * n : 0.0);
+  const double t_hbm = bytes / 5.0e12;
+  return std::max(t_mfma, t_hbm) + (slices > 1 ? 6e-6 : 0.0);
+}
+
+Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
+                 const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants], int mode, int oa, int ob, int oc,
+                 int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b) {
+  Plan best;
+  const int ncu = std::max(1, st->num_cus);
+  // 1. explicit override (measurement and tuning): MTK_SHGEMM_PLAN=<variant>,<slices>
+  if (st->forced_variant >= 0 && tab[st->forced_variant].fn != nullptr) {
+    best.variant = st->forced_variant;
+    best.slices = shgemm_amd::variant_is_split(best.variant)
+                      ? (int)std::min<int64_t>(st->forced_slices, max_slices(tab[best.variant], m, n, k, st->ws_bytes))
+                      : 1;
+    if (best.slices < 1) best.slices = 1;
+    best.level = best.slices > 1 ? detail::P0 : shgemm_amd::variant_level(best.variant);
+    best.source = " (forced)";
+    return best;
+  }
+  // 2. kernel level fixed by the caller (reference shgemm.cu:655): no split-K
+  if (handle.fixed_lernel_level < detail::num_levels) {
+    best.variant = handle.fixed_lernel_level == detail::P1 ? shgemm_amd::V_P1 : shgemm_amd::V_P0;
+    best.level = handle.fixed_lernel_level;
+    best.source = " (fixed level)";
+    return best;
+  }
+  // 3. cost model over every variant and split count
+  double best_t = 1e30;
+  for (int v = 0; v < shgemm_amd::kNumVariants; ++v) {
+    const auto& e = tab[v];
+    if (e.fn == nullptr) continue;
+    const bool split = shgemm_amd::variant_is_split(v);
+    const int64_t smax = split ? max_slices(e, m, n, k, st->ws_bytes) : 1;
+    for (int64_t s = split ? 2 : 1; s <= smax; s *= 2) {
+      const double t = plan_cost(e, v, mode, s, m, n, k, ncu);
+      if (t < best_t) {
+        best_t = t;
+        best.variant = v;
+        best.slices = (int)s;
+      }
+    }
+  }
+  best.level = best.slices > 1 ? detail::P0 : shgemm_amd::variant_level(best.variant);
+  // tile-sweep candidate or deep-prefetch kernel for the large-tile fp16 path
+  if (best.slices == 1 && best.variant == shgemm_amd::V_P1) {
+    if (st->tune_p1 >= 0 && st->table.tune[st->tune_p1].mode == mode && oa == shgemm_amd::OP_N &&
+        ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N) {
+      best.entry = &st->table.tune[st->tune_p1];
+      best.source = " (tile sweep candidate)";
+    } else {
+      const auto& bd = st->table.shgemm_bd[oa][oc];
+      if (st->use_bd && bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b &&
+          k % bd.bk == 0 && (oa == shgemm_amd::OP_N || vec_a)) {
+        best.entry = &bd;
+        best.source = " (deep prefetch)";
+      }
+    }
+  }
+  return best;
+}
+
 }  // namespace
 
 void create(shgemmHandle_t& handle) {
@@ -118,8 +218,7 @@ void create(shgemmHandle_t& handle) {
         for (auto& c : b)
           for (auto& d : c)
             for (auto& e : d)
-              for (auto& f : e)
-                if (f.fn) set_lds_limit(f);
+              if (e.fn) set_lds_limit(e);
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
     st->ws_bytes = kWorkspaceBytes;
   } catch (...) {
@@ -138,6 +237,14 @@ void create(shgemmHandle_t& handle) {
       if (e.fn) set_lds_limit(e);
   // Opt-in: at 176 VGPRs it runs 2 wavefronts per SIMD and measured slower
   // than the default large tile at 16384^3 (389 vs 473 TFLOP/s effective).
+  const char* fp = std::getenv("MTK_SHGEMM_PLAN");
+  if (fp != nullptr) {
+    int v = -1, sl = 1;
+    if (std::sscanf(fp, "%d,%d", &v, &sl) >= 1 && v >= 0 && v < shgemm_amd::kNumVariants) {
+      st->forced_variant = v;
+      st->forced_slices = std::max(1, sl);
+    }
+  }
   const char* bd = std::getenv("MTK_SHGEMM_BD");
   st->use_bd = (bd != nullptr && bd[0] == '1');
   // Per-row exponent prescaling of the fp16 compute type is on unless
@@ -238,73 +345,31 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   }
 
   const auto& tab = st->table.shgemm[oa][ob][oc][mode];
-  const int ncu = std::max(1, st->num_cus);
-
-  // Split-K when the output is too small to fill the GPU and k is long
-  // (reference shgemm.cu:532 uses m*n <= 884,736 && k >= 4096 && beta == 0;
-  // here beta is applied by the reduction, so beta != 0 is allowed too).
-  {
-    const auto& e = tab[shgemm_amd::LEVEL_P1][1];
-    const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
-    if (tiles < ncu && k >= 4096) {
-      // Slices: enough workgroups for two per CU, each slice at least 8 k-tiles,
-      // all partials inside the workspace.  Depends on the shape and CU count only.
-      int64_t slices = 1;
-      while (tiles * slices < 2 * (int64_t)ncu && (int64_t)k / (slices * 2) >= 8 * e.bk &&
-             (slices * 2) * elems * (int64_t)sizeof(float) <= (int64_t)st->ws_bytes)
-        slices *= 2;
-      if (slices > 1) {
-        const int64_t kslice = (((int64_t)k + slices - 1) / slices + e.bk - 1) / e.bk * e.bk;
-        const int64_t used = ((int64_t)k + kslice - 1) / kslice;
-        p.k_slice = (int)kslice;
-        p.ws = st->ws;
-        p.tiles_m = (int)((m + e.bm - 1) / e.bm);
-        p.tiles_n = (int)((n + e.bn - 1) / e.bn);
-        p.group_m = group_rows(p.tiles_m);
-        if (handle.debug_mode)
-          std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] split-K tile=%dx%d slices=%lld\n", m, n,
-                      k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn,
-                      (long long)used);
-        launch(e, dim3((unsigned)tiles, (unsigned)used), p, stream);
-        check_hip(shgemm_amd::launch_splitk_reduce(st->ws, (int)used, elems, inner, alpha, beta, c_ptr,
-                                                   (int64_t)ldc, stream),
-                  "split-K reduction");
-        return detail::P0;
-      }
-    }
-  }
-
-  // Tile level (reference shgemm.cu:643-656): the large tiles when they give
-  // at least two waves of workgroups, else the small ones.
-  int level = detail::P1;
-  if (handle.fixed_lernel_level < detail::num_levels) {
-    level = handle.fixed_lernel_level;
-  } else {
-    const auto& e = tab[shgemm_amd::LEVEL_P1][0];
-    const int64_t tiles = (int64_t)((m + e.bm - 1) / e.bm) * (int64_t)((n + e.bn - 1) / e.bn);
-    const int64_t filling = 2 * (int64_t)ncu;  // two workgroups per CU
-    level = tiles >= 2 * filling ? detail::P1 : detail::P0;
-  }
-  const bool tuned = st->tune_p1 >= 0 && level == detail::P1 && st->table.tune[st->tune_p1].mode == mode &&
-                     oa == shgemm_amd::OP_N && ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N;
-  // Deep-prefetch kernel: prescaled fp16, op_b = n, large tiles, full k-tiles,
-  // B rows 16-byte aligned; A needs 16-byte alignment only when k-contiguous.
-  const auto& bd = st->table.shgemm_bd[oa][oc];
-  const bool use_bd = !tuned && st->use_bd && bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S &&
-                      level == detail::P1 && ob == shgemm_amd::OP_N && p.vec_b && (k % bd.bk) == 0 &&
-                      (oa == shgemm_amd::OP_N || p.vec_a);
-  const auto& e = tuned ? st->table.tune[st->tune_p1] : use_bd ? bd : tab[level][0];
+  const Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
+                                p.vec_b != 0);
+  const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
   p.group_m = group_rows(p.tiles_m);
   const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
   if (tiles > INT_MAX) throw error("too many output tiles");
   if (handle.debug_mode)
-    std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d kernel_level=%d%s\n", m, n, k,
-                op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk, level,
-                scaled ? " prescaled" : "");
-  launch(e, dim3((unsigned)tiles), p, stream);
-  return static_cast<detail::kernel_level>(level);
+    std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d slices=%d kernel_level=%d%s%s\n", m,
+                n, k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk,
+                plan.slices, plan.level, scaled ? " prescaled" : "", plan.source);
+  if (plan.slices > 1) {
+    const int64_t kslice = (((int64_t)k + plan.slices - 1) / plan.slices + e.bk - 1) / e.bk * e.bk;
+    const int64_t used = ((int64_t)k + kslice - 1) / kslice;
+    p.k_slice = (int)kslice;
+    p.ws = st->ws;
+    launch(e, dim3((unsigned)tiles, (unsigned)used), p, stream);
+    check_hip(shgemm_amd::launch_splitk_reduce(st->ws, (int)used, elems, inner, alpha, beta, c_ptr, (int64_t)ldc,
+                                               stream),
+              "split-K reduction");
+  } else {
+    launch(e, dim3((unsigned)tiles), p, stream);
+  }
+  return static_cast<detail::kernel_level>(plan.level);
 }
 
 void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b, const std::size_t m,

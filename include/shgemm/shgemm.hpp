@@ -126,10 +126,71 @@ void sgemm_fp32(
     const operation_t op_c = op_n);
 
 // Per-row power-of-two prescaling of the FP32 operand for the fp16 compute
-// type.  Off by default (the reference's fp16 mode has FP16's exponent range).
-// When on, one extra pass over A computes each row's exponent; the split runs
-// on A * 2^-e(row) and the row of C is scaled back exactly.
+// type.  On by default (MTK_SHGEMM_EXP_SCALING=0 turns it off at create, which
+// gives the reference's fp16 mode with FP16's exponent range).  When on, one
+// extra pass over A computes each row's exponent; the split runs on
+// A * 2^-e(row) and the row of C is scaled back exactly.
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
+
+// ---- Kernel plans --------------------------------------------------------
+// A plan is a kernel variant plus a k-slice count.  Variants:
+//   0 small tile (level P0), 1 large tile (P1), 2 128x128 split-K,
+//   3 large-tile split-K, 4 small-tile split-K, 5 large tile with the
+//   three-stage direct-to-LDS pipeline (prescaled fp16, op_b = op_n, aligned
+//   B, k a multiple of 32; otherwise it runs as 1).
+// shgemm picks, in this order: a forced plan, a fixed kernel level, the
+// nearest entry of the measured plan table (written by the autotuner,
+// shgemm_amd/tune.py), and otherwise a cost model.
+enum { plan_variant_bd = 5, num_plan_variants = 6 };
+enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
+struct plan_t {
+  int variant = 1;
+  int slices = 1;
+  plan_source source = plan_cost_model;
+};
+
+// Every later shgemm call on this handle runs `variant` with `slices`
+// k-slices (clamped to what the workspace and k allow); variant < 0 clears.
+void force_plan(shgemmHandle_t& handle, const int variant, const int slices);
+// Replaces the handle's plan table with the one in `path`; nullptr or ""
+// leaves the handle with no table (cost model only).  create() loads
+// $MTK_SHGEMM_PLANS, else plans_gfx950.tsv beside the library, if present.
+void load_plans(shgemmHandle_t& handle, const char* path);
+// The plan shgemm would use for these arguments (nothing is launched).
+plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
+                  const std::size_t m, const std::size_t n, const std::size_t k, const tc_t compute_type,
+                  const operation_t op_c = op_n);
+
+// ---- Grouped SHGEMM ------------------------------------------------------
+// Many independent problems C_i = alpha_i op(A_i) op(B_i) + beta_i C_i that
+// share op_a, op_b, op_c and compute_type, run by persistent kernels that
+// take (problem, k-slice, tile) work units from a queue, longest first.  Each
+// problem gets a tile class (0 small, 1 128x128, 2 the large tile) and a
+// k-slice count from the plan table's grouped entries (or the arguments).
+// create_group validates every problem, allocates the group's descriptors,
+// row-exponent scratch and split-K partials, and captures the whole sequence
+// (row-exponent pass, one launch per tile class used, split-K reduction) in
+// one HIP graph; run_group replays it on `stream`.  Results are bit-identical
+// to running each problem alone with the same tile depth and slice count.
+struct group_problem {
+  std::size_t m = 0, n = 0, k = 0;
+  const float* A = nullptr;
+  std::size_t lda = 0;
+  const half* B = nullptr;
+  std::size_t ldb = 0;
+  float* C = nullptr;
+  std::size_t ldc = 0;
+  float alpha = 1.0f, beta = 0.0f;
+};
+struct group_t;
+group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, const operation_t op_b,
+                      const group_problem* problems, const std::size_t count, const tc_t compute_type,
+                      const operation_t op_c = op_n, const int* tile_class = nullptr, const int* slices = nullptr);
+// use_graph = false enqueues the same launches one by one (no graph).
+void run_group(group_t* group, hipStream_t stream, const bool use_graph = true);
+void destroy_group(group_t* group);
+// Per-problem choices (tile class, slices) and the group's work-unit count.
+void group_plan(const group_t* group, int* tile_class, int* slices, std::size_t* units);
 
 }  // namespace shgemm
 }  // namespace mtk

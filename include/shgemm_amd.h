@@ -75,6 +75,54 @@ int shgemm_amd_sgemm_fp32(shgemm_amd_handle* h, int op_a, int op_b,
  * FP32 operand for the fp16 compute type (widens its exponent range). */
 int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on);
 
+/* ---- Kernel plans ---------------------------------------------------------
+ * Replace the reference's static per-architecture tile tables and its
+ * level/split heuristic (src/shgemm.cu:427-489, 532, 643-656) with a measured
+ * plan table (written by shgemm_amd/tune.py) and a cost model.
+ * Plan variants: 0 small tile (P0), 1 large tile (P1), 2 128x128 split-K,
+ * 3 large-tile split-K, 4 small-tile split-K, 5 large tile with the
+ * three-stage direct-to-LDS pipeline (runs as 1 where it does not apply).
+ * Plan sources (query): 0 cost model, 1 plan table, 2 forced, 3 fixed level. */
+#define SHGEMM_AMD_PLAN_BD 5
+/* Force variant/slices for every later call on h (variant < 0 clears). */
+int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
+/* Replace h's plan table with the file at path (NULL or "": no table). */
+int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path);
+/* The plan shgemm would run for these arguments, for aligned operands. */
+int shgemm_amd_query_plan(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k,
+                          int compute_type, int op_c, int* variant, int* slices, int* source);
+/* Parse a plan table file on the host (no GPU needed); *rows gets its row count. */
+int shgemm_amd_check_plan_file(const char* path, int* rows);
+
+/* ---- Grouped SHGEMM ---------------------------------------------------------
+ * No counterpart in the reference (one problem per call there,
+ * include/shgemm/shgemm.hpp:105-119).  Each problem follows shgemm's
+ * semantics; all share op_a, op_b, op_c and compute_type.  create plans the
+ * problems (tile class 0 small / 1 128x128 / 2 large, and k-slices: from
+ * tile_class/slices when not NULL, else the plan table), allocates the
+ * group's device scratch and captures one HIP graph; run replays it (or, with
+ * use_graph = 0, enqueues the same launches one by one) on `stream`.  Runs of
+ * one group must not overlap (they share its scratch). */
+typedef struct shgemm_amd_problem {
+  size_t m, n, k;
+  const float* a;
+  size_t lda;
+  const uint16_t* b;  /* FP16 bits */
+  size_t ldb;
+  float* c;
+  size_t ldc;
+  float alpha, beta;
+} shgemm_amd_problem;
+typedef struct shgemm_amd_group shgemm_amd_group;
+int shgemm_amd_group_create(shgemm_amd_handle* h, int op_a, int op_b, int op_c, int compute_type,
+                            const shgemm_amd_problem* problems, size_t count,
+                            const int* tile_class, const int* slices, shgemm_amd_group** out);
+int shgemm_amd_group_run(shgemm_amd_group* g, void* stream, int use_graph);
+/* Per-problem tile class and slice count (-1 / 0 for problems with no MFMA
+ * work), and the total number of work units. Arrays may be NULL. */
+int shgemm_amd_group_plan(shgemm_amd_group* g, int* tile_class, int* slices, size_t* units);
+int shgemm_amd_group_destroy(shgemm_amd_group* g);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* shgemm_amd_last_error(void);
 

@@ -78,6 +78,15 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_hsgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
     L.shgemm_amd_sgemm_fp32.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32]
     L.shgemm_amd_set_exponent_scaling.argtypes = [vp, i32]
+    L.shgemm_amd_force_plan.argtypes = [vp, i32, i32]
+    L.shgemm_amd_load_plans.argtypes = [vp, ctypes.c_char_p]
+    L.shgemm_amd_query_plan.argtypes = [vp, i32, i32, sz, sz, sz, i32, i32, ip, ip, ip]
+    L.shgemm_amd_check_plan_file.argtypes = [ctypes.c_char_p, ip]
+    L.shgemm_amd_group_create.argtypes = [vp, i32, i32, i32, i32, ctypes.POINTER(Problem), sz, ip, ip,
+                                          ctypes.POINTER(vp)]
+    L.shgemm_amd_group_run.argtypes = [vp, vp, i32]
+    L.shgemm_amd_group_plan.argtypes = [vp, ip, ip, ctypes.POINTER(sz)]
+    L.shgemm_amd_group_destroy.argtypes = [vp]
     L.shgemm_amd_last_error.restype = ctypes.c_char_p
     L.shgemm_amd_version.restype = ctypes.c_char_p
     for name in exported_symbols_in_header():
@@ -109,6 +118,16 @@ def _dptr(x) -> int | None:
 
 def _fptr(v: float):
     return ctypes.pointer(ctypes.c_float(float(v)))
+
+
+class Problem(ctypes.Structure):
+    """One problem of a group (shgemm_amd_problem in include/shgemm_amd.h)."""
+
+    _fields_ = [("m", ctypes.c_size_t), ("n", ctypes.c_size_t), ("k", ctypes.c_size_t),
+                ("a", ctypes.c_void_p), ("lda", ctypes.c_size_t),
+                ("b", ctypes.c_void_p), ("ldb", ctypes.c_size_t),
+                ("c", ctypes.c_void_p), ("ldc", ctypes.c_size_t),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float)]
 
 
 class shgemmHandle_t:
@@ -190,6 +209,82 @@ def sgemm_fp32(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
     """Plain FP32 SGEMM (the comparator), same conventions."""
     _check(load_library().shgemm_amd_sgemm_fp32(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha), _dptr(a),
                                                 lda, _dptr(b), ldb, _fptr(beta), _dptr(c), ldc, int(op_c)))
+
+
+# ---- kernel plans -------------------------------------------------------------
+PLAN_VARIANTS = {0: "P0", 1: "P1", 2: "SK128", 3: "P1-SK", 4: "P0-SK", 5: "P1-3stage"}
+PLAN_SOURCES = {0: "cost model", 1: "table", 2: "forced", 3: "fixed level"}
+
+
+def force_plan(handle, variant: int, slices: int = 1) -> None:
+    """Run every later call on `handle` with this plan (variant < 0 clears)."""
+    _check(load_library().shgemm_amd_force_plan(handle.ptr, int(variant), int(slices)))
+
+
+def load_plans(handle, path) -> None:
+    """Replace the handle's plan table (None or "": no table, cost model only)."""
+    raw = None if path is None else str(path).encode()
+    _check(load_library().shgemm_amd_load_plans(handle.ptr, raw))
+
+
+def query_plan(handle, op_a, op_b, m, n, k, compute_type, op_c=op_n) -> tuple[int, int, int]:
+    """(variant, slices, source) that shgemm would use for these arguments."""
+    v, s, src = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(load_library().shgemm_amd_query_plan(handle.ptr, int(op_a), int(op_b), m, n, k, int(compute_type),
+                                                int(op_c), ctypes.byref(v), ctypes.byref(s), ctypes.byref(src)))
+    return v.value, s.value, src.value
+
+
+def check_plan_file(path) -> int:
+    """Parse a plan table on the host; returns its row count or raises."""
+    rows = ctypes.c_int()
+    _check(load_library().shgemm_amd_check_plan_file(str(path).encode(), ctypes.byref(rows)))
+    return rows.value
+
+
+# ---- grouped SHGEMM -------------------------------------------------------------
+class Group:
+    """A captured grouped SHGEMM (shgemm_amd_group_*).  `problems` is a list of
+    dicts or tuples (m, n, k, A, lda, B, ldb, C, ldc, alpha, beta) with device
+    tensors or int addresses; operands must stay alive while the group is used."""
+
+    def __init__(self, handle, op_a, op_b, problems, compute_type=fp16, op_c=op_n, tile_class=None, slices=None):
+        n = len(problems)
+        arr = (Problem * max(n, 1))()
+        keys = ("m", "n", "k", "a", "lda", "b", "ldb", "c", "ldc", "alpha", "beta")
+        for i, q in enumerate(problems):
+            d = dict(zip(keys, q)) if not isinstance(q, dict) else q
+            arr[i] = Problem(d["m"], d["n"], d["k"], _dptr(d["a"]), d["lda"], _dptr(d["b"]), d["ldb"],
+                             _dptr(d["c"]), d["ldc"], float(d.get("alpha", 1.0)), float(d.get("beta", 0.0)))
+        tc = (ctypes.c_int * n)(*tile_class) if tile_class is not None else None
+        sl = (ctypes.c_int * n)(*slices) if slices is not None else None
+        self._g = ctypes.c_void_p()
+        self.count = n
+        _check(load_library().shgemm_amd_group_create(handle.ptr, int(op_a), int(op_b), int(op_c), int(compute_type),
+                                                      arr, n, tc, sl, ctypes.byref(self._g)))
+
+    def run(self, stream=None, use_graph: bool = True) -> None:
+        raw = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+        _check(load_library().shgemm_amd_group_run(self._g, raw, 1 if use_graph else 0))
+
+    def plan(self) -> tuple[list[int], list[int], int]:
+        """Per-problem tile class and slice count, and the total work units."""
+        tc = (ctypes.c_int * self.count)()
+        sl = (ctypes.c_int * self.count)()
+        units = ctypes.c_size_t()
+        _check(load_library().shgemm_amd_group_plan(self._g, tc, sl, ctypes.byref(units)))
+        return list(tc), list(sl), units.value
+
+    def destroy(self) -> None:
+        if self._g:
+            _check(load_library().shgemm_amd_group_destroy(self._g))
+            self._g = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
 
 
 def version() -> str:

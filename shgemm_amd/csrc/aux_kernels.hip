@@ -11,10 +11,9 @@ namespace shgemm_amd {
 // ws element e is C element (e % inner, e / inner) in C's own storage order
 // (inner = m for column-major C, n for transposed C).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) splitk_reduce_kernel(const float* __restrict__ ws, int slices, int64_t elems,
-                                                             int inner, float alpha, float beta,
-                                                             float* __restrict__ c, int64_t ldc) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void splitk_reduce_elem(const float* __restrict__ ws, int slices, int64_t elems, int inner,
+                                                   float alpha, float beta, float* __restrict__ c, int64_t ldc,
+                                                   int64_t e) {
   if (e >= elems) return;
   float s = 0.0f;
   for (int i = 0; i < slices; ++i) s += ws[(int64_t)i * elems + e];
@@ -23,13 +22,23 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(const float* __restr
   *cp = (beta == 0.0f) ? alpha * s : alpha * s + beta * *cp;
 }
 
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const float* __restrict__ ws, int slices, int64_t elems,
+                                                             int inner, float alpha, float beta,
+                                                             float* __restrict__ c, int64_t ldc) {
+  splitk_reduce_elem(ws, slices, elems, inner, alpha, beta, c, ldc, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
 // beta-only update (k == 0): C = beta * C, or zero when beta == 0.
-__global__ void __launch_bounds__(256) scale_c_kernel(int64_t elems, int inner, float beta, float* __restrict__ c,
-                                                       int64_t ldc) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void scale_c_elem(int64_t elems, int inner, float beta, float* __restrict__ c, int64_t ldc,
+                                             int64_t e) {
   if (e >= elems) return;
   float* cp = c + (e % inner) + (e / inner) * ldc;
   *cp = (beta == 0.0f) ? 0.0f : beta * *cp;
+}
+
+__global__ void __launch_bounds__(256) scale_c_kernel(int64_t elems, int inner, float beta, float* __restrict__ c,
+                                                       int64_t ldc) {
+  scale_c_elem(elems, inner, beta, c, ldc, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -38,8 +47,8 @@ __global__ void __launch_bounds__(256) scale_c_kernel(int64_t elems, int inner, 
 //   row_exp[r] = ilogb(max |A[r,:]|) - 14, so A[r,:] * 2^-row_exp lies below
 //   2^15 (no FP16 overflow in hi) and its largest entries sit in [2^14, 2^15).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t* __restrict__ p, int n, uint32_t v) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
@@ -52,14 +61,14 @@ __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b 
 // With `vec` a row-quad at one k is one 16-byte load (rows 4-aligned,
 // lda % 4 == 0, 16-byte base).  grid.x covers row-quads in groups of rq,
 // grid.y covers k in chunks of kchunk; the max is order-independent.
-__global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                              int kchunk, int rq, int vec,
-                                                              uint32_t* __restrict__ row_max) {
+__device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a, int64_t lda, int m, int k,
+                                                     int kchunk, int rq, int vec, uint32_t* __restrict__ row_max,
+                                                     int bx, int by) {
   const int q = threadIdx.x % rq, kl = threadIdx.x / rq, klanes = 256 / rq;
-  const int r4 = (blockIdx.x * rq + q) * 4;
+  const int r4 = (bx * rq + q) * 4;
   uint32_t mx[4] = {0u, 0u, 0u, 0u};
   if (r4 < m && kl < klanes) {
-    const int k0 = blockIdx.y * kchunk, k1 = min(k, k0 + kchunk);
+    const int k0 = by * kchunk, k1 = min(k, k0 + kchunk);
     if (vec && r4 + 4 <= m) {
 #pragma unroll 4
       for (int kk = k0 + kl; kk < k1; kk += klanes) {
@@ -88,13 +97,18 @@ __global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __rest
   }
 }
 
+__global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
+                                                              int kchunk, int rq, int vec,
+                                                              uint32_t* __restrict__ row_max) {
+  rowmax_rcontig_block(a, lda, m, k, kchunk, rq, vec, row_max, blockIdx.x, blockIdx.y);
+}
+
 // A stored with k contiguous (op_t): element (r, k) at a[r*lda + k].
 // One workgroup per (row, k chunk): 16-byte loads when `vec`, a block
 // reduction, then one atomic.
-__global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                              int kchunk, int vec, uint32_t* __restrict__ row_max) {
-  const int r = blockIdx.x;
-  const int k0 = blockIdx.y * kchunk, k1 = min(k, k0 + kchunk);
+__device__ __forceinline__ void rowmax_kcontig_block(const float* __restrict__ a, int64_t lda, int k, int kchunk,
+                                                     int vec, uint32_t* __restrict__ row_max, int r, int by) {
+  const int k0 = by * kchunk, k1 = min(k, k0 + kchunk);
   const float* row = a + (int64_t)r * lda;
   uint32_t mx = 0;
   if (vec) {
@@ -119,8 +133,15 @@ __global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __rest
   if (threadIdx.x == 0) atomicMax(&row_max[r], red[0]);
 }
 
-__global__ void __launch_bounds__(256) rowexp_kernel(const uint32_t* __restrict__ row_max, int m, int* __restrict__ row_exp) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
+                                                              int kchunk, int vec, uint32_t* __restrict__ row_max) {
+  (void)m;
+  rowmax_kcontig_block(a, lda, k, kchunk, vec, row_max, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) rowexp_kernel(const uint32_t* __restrict__ row_max, int64_t m,
+                                                      int* __restrict__ row_exp) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= m) return;
   const uint32_t b = row_max[r];
   int e = 0;
@@ -148,36 +169,104 @@ hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_
   return hipGetLastError();
 }
 
+RowmaxGeometry rowmax_geometry(int op_a, int m, int k) {
+  RowmaxGeometry g{};
+  if (op_a == OP_N) {
+    // rq row-quads per workgroup (a power of two up to 256), the rest of the
+    // 256 threads spread over k; about 64 loads per thread; grid.y <= 65535
+    const int quads = (m + 3) / 4;
+    g.rq = 1;
+    while (g.rq < 256 && g.rq < quads) g.rq *= 2;
+    const int klanes = 256 / g.rq;
+    g.kchunk = std::max(64 * klanes, (int)((k + 65534) / 65535));
+    g.bx = (quads + g.rq - 1) / g.rq;
+  } else {
+    g.rq = 0;
+    g.kchunk = std::max(4096, (int)((k + 65534) / 65535));
+    g.bx = m;
+  }
+  g.by = k > 0 ? (k + g.kchunk - 1) / g.kchunk : 0;
+  return g;
+}
+
 hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
                                 int* row_exp, hipStream_t s) {
-  const unsigned rb = (unsigned)((m + 255) / 256);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(rb), dim3(256), 0, s, row_max, m, 0u);
-  hipError_t err = hipGetLastError();
+  hipError_t err = launch_fill_u32(row_max, m, 0u, s);
   if (err != hipSuccess) return err;
   const bool aligned16 = (reinterpret_cast<uintptr_t>(a) % 16) == 0 && (lda % 4) == 0;
   if (k > 0) {
-    if (op_a == OP_N) {
-      // rq row-quads per workgroup (a power of two up to 256), the rest of the
-      // 256 threads spread over k; about 64 loads per thread; grid.y <= 65535
-      const int quads = (m + 3) / 4;
-      int rq = 1;
-      while (rq < 256 && rq < quads) rq *= 2;
-      const int klanes = 256 / rq;
-      const int kchunk = std::max(64 * klanes, (int)((k + 65534) / 65535));
-      const unsigned kb = (unsigned)((k + kchunk - 1) / kchunk);
-      const unsigned rbq = (unsigned)((quads + rq - 1) / rq);
-      hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3(rbq, kb), dim3(256), 0, s, a, lda, m, k, kchunk, rq,
-                         aligned16 ? 1 : 0, row_max);
-    } else {
-      const int kchunk = std::max(4096, (int)((k + 65534) / 65535));
-      const unsigned kb = (unsigned)((k + kchunk - 1) / kchunk);
-      hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)m, kb), dim3(256), 0, s, a, lda, m, k, kchunk,
-                         aligned16 ? 1 : 0, row_max);
-    }
+    const RowmaxGeometry g = rowmax_geometry(op_a, m, k);
+    if (op_a == OP_N)
+      hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
+                         g.kchunk, g.rq, aligned16 ? 1 : 0, row_max);
+    else
+      hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
+                         g.kchunk, aligned16 ? 1 : 0, row_max);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
-  hipLaunchKernelGGL(rowexp_kernel, dim3(rb), dim3(256), 0, s, row_max, m, row_exp);
+  return launch_row_exp(row_max, m, row_exp, s);
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rowexp_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, row_max, rows, row_exp);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Grouped forms: one launch over the blocks of every problem's segment.
+// Block b belongs to the last segment whose first block is <= b.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_segment(const int* __restrict__ begin, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int OPA>
+__global__ void __launch_bounds__(256) grouped_rowmax_kernel(const RowmaxSeg* __restrict__ segs,
+                                                              const int* __restrict__ begin, int nseg) {
+  const int b = blockIdx.x;
+  const int i = find_segment(begin, nseg, b);
+  const RowmaxSeg sg = segs[i];
+  const int local = b - sg.block_begin;
+  const int bx = local % sg.bx, by = local / sg.bx;
+  if constexpr (OPA == OP_N) rowmax_rcontig_block(sg.a, sg.lda, sg.m, sg.k, sg.kchunk, sg.rq, sg.vec, sg.row_max, bx, by);
+  else rowmax_kcontig_block(sg.a, sg.lda, sg.k, sg.kchunk, sg.vec, sg.row_max, bx, by);
+}
+
+__global__ void __launch_bounds__(256) grouped_reduce_kernel(const ReduceSeg* __restrict__ segs,
+                                                              const int* __restrict__ begin, int nseg) {
+  const int b = blockIdx.x;
+  const int i = find_segment(begin, nseg, b);
+  const ReduceSeg sg = segs[i];
+  const int64_t e = (int64_t)(b - sg.block_begin) * 256 + threadIdx.x;
+  if (sg.slices > 0) splitk_reduce_elem(sg.ws, sg.slices, sg.elems, sg.inner, sg.alpha, sg.beta, sg.c, sg.ldc, e);
+  else scale_c_elem(sg.elems, sg.inner, sg.beta, sg.c, sg.ldc, e);  // k == 0 or alpha == 0
+}
+
+hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,
+                                 hipStream_t s) {
+  if (nseg <= 0 || blocks <= 0) return hipSuccess;
+  if (op_a == OP_N) hipLaunchKernelGGL(grouped_rowmax_kernel<OP_N>, dim3((unsigned)blocks), dim3(256), 0, s, segs, begin, nseg);
+  else hipLaunchKernelGGL(grouped_rowmax_kernel<OP_T>, dim3((unsigned)blocks), dim3(256), 0, s, segs, begin, nseg);
+  return hipGetLastError();
+}
+
+hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int nseg, int64_t blocks, hipStream_t s) {
+  if (nseg <= 0 || blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grouped_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, segs, begin, nseg);
   return hipGetLastError();
 }
 

@@ -3,13 +3,19 @@
 // records the message for shgemm_amd_last_error().
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/shgemm/shgemm.hpp"
 #include "../../include/shgemm_amd.h"
 #include "errors.hpp"
+#include "plans.hpp"
 
 struct shgemm_amd_handle {
   mtk::shgemm::shgemmHandle_t h;
+};
+
+struct shgemm_amd_group {
+  mtk::shgemm::group_t* g = nullptr;
 };
 
 namespace {
@@ -147,6 +153,98 @@ int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on) {
   return guarded([&] {
     need(h);
     mtk::shgemm::set_exponent_scaling(h->h, on != 0);
+  });
+}
+
+int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::force_plan(h->h, variant, slices);
+  });
+}
+
+int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::load_plans(h->h, path);
+  });
+}
+
+int shgemm_amd_query_plan(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k, int compute_type,
+                          int op_c, int* variant, int* slices, int* source) {
+  return guarded([&] {
+    need(h);
+    const auto pl =
+        mtk::shgemm::query_plan(h->h, to_op(op_a), to_op(op_b), m, n, k, to_tc(compute_type), to_op(op_c));
+    if (variant) *variant = pl.variant;
+    if (slices) *slices = pl.slices;
+    if (source) *source = static_cast<int>(pl.source);
+  });
+}
+
+int shgemm_amd_check_plan_file(const char* path, int* rows) {
+  return guarded([&] {
+    if (path == nullptr) throw mtk::shgemm::error("path is null");
+    std::vector<shgemm_amd::PlanRow> v;
+    std::string err;
+    if (!shgemm_amd::parse_plan_file(path, v, err)) throw mtk::shgemm::error(err);
+    if (rows) *rows = (int)v.size();
+  });
+}
+
+int shgemm_amd_group_create(shgemm_amd_handle* h, int op_a, int op_b, int op_c, int compute_type,
+                            const shgemm_amd_problem* problems, size_t count, const int* tile_class,
+                            const int* slices, shgemm_amd_group** out) {
+  return guarded([&] {
+    need(h);
+    if (out == nullptr) throw mtk::shgemm::error("out is null");
+    if (problems == nullptr && count > 0) throw mtk::shgemm::error("problems is null");
+    std::vector<mtk::shgemm::group_problem> v(count);
+    for (size_t i = 0; i < count; ++i) {
+      const shgemm_amd_problem& q = problems[i];
+      v[i].m = q.m;
+      v[i].n = q.n;
+      v[i].k = q.k;
+      v[i].A = q.a;
+      v[i].lda = q.lda;
+      v[i].B = reinterpret_cast<const half*>(q.b);
+      v[i].ldb = q.ldb;
+      v[i].C = q.c;
+      v[i].ldc = q.ldc;
+      v[i].alpha = q.alpha;
+      v[i].beta = q.beta;
+    }
+    auto* g = new shgemm_amd_group();
+    try {
+      g->g = mtk::shgemm::create_group(h->h, to_op(op_a), to_op(op_b), v.data(), count, to_tc(compute_type),
+                                       to_op(op_c), tile_class, slices);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int shgemm_amd_group_run(shgemm_amd_group* g, void* stream, int use_graph) {
+  return guarded([&] {
+    if (g == nullptr) throw mtk::shgemm::error("group is null");
+    mtk::shgemm::run_group(g->g, static_cast<hipStream_t>(stream), use_graph != 0);
+  });
+}
+
+int shgemm_amd_group_plan(shgemm_amd_group* g, int* tile_class, int* slices, size_t* units) {
+  return guarded([&] {
+    if (g == nullptr) throw mtk::shgemm::error("group is null");
+    mtk::shgemm::group_plan(g->g, tile_class, slices, units);
+  });
+}
+
+int shgemm_amd_group_destroy(shgemm_amd_group* g) {
+  return guarded([&] {
+    if (g == nullptr) return;
+    mtk::shgemm::destroy_group(g->g);
+    delete g;
   });
 }
 

@@ -49,11 +49,9 @@ struct GemmParams {
 // round-robin over the 8 XCDs; remap so each XCD owns a contiguous run of
 // tiles, then walk that run in groups of `group_m` tile rows so the tiles in
 // flight on one XCD share A and B panels in its L2.
-__device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& tm, int& tn) {
-  const int nb = p.tiles_m * p.tiles_n;
-  const int q = nb >> 3, r = nb & 7;
-  const int x = bid & 7, i = bid >> 3;
-  const int t = x * q + (x < r ? x : r) + i;
+// Linear tile index t -> (tm, tn): groups of `group_m` tile rows, column by
+// column inside a group.
+__device__ __forceinline__ void grouped_order(const GemmParams& p, int t, int& tm, int& tn) {
   const int per_group = p.group_m * p.tiles_n;
   const int g = t / per_group;
   const int first = g * p.group_m;
@@ -61,6 +59,13 @@ __device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& t
   const int local = t - g * per_group;
   tm = first + local % rows;
   tn = local / rows;
+}
+
+__device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& tm, int& tn) {
+  const int nb = p.tiles_m * p.tiles_n;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, i = bid >> 3;
+  grouped_order(p, x * q + (x < r ? x : r) + i, tm, tn);
 }
 
 // LDS operand layout: row-major 16-byte slots, the chunk index of each row

@@ -3,7 +3,7 @@
 // combination so the variants compile in parallel.
 #pragma once
 #include "registry.hpp"
-#include "shgemm_kernel.hpp"
+#include "grouped_kernel.hpp"
 
 namespace shgemm_amd {
 
@@ -19,8 +19,24 @@ inline void put_shgemm(KernelTable& t, int variant) {
   e.threads = CFG::NT;
 }
 
+template <int OPA, int OPB, int OPC, int MODE, class CFG>
+inline void put_grouped(KernelTable& t, int cls) {
+  KernelEntry& e = t.grouped[OPA][OPB][OPC][MODE][cls];
+  e.fn = reinterpret_cast<const void*>(&shgemm_grouped_kernel<OPA, OPB, OPC, MODE, CFG>);
+  e.lds = CFG::template lds_bytes<MODE>();
+  e.bm = CFG::BM;
+  e.bn = CFG::BN;
+  e.bk = CFG::BK;
+  e.mode = MODE;
+  e.threads = CFG::NT;
+}
+
 template <int OPA, int OPB, int OPC, int MODE>
 inline void put_shgemm_opc(KernelTable& t) {
+  using G = GroupCfg<MODE>;
+  put_grouped<OPA, OPB, OPC, MODE, typename G::C0>(t, 0);
+  put_grouped<OPA, OPB, OPC, MODE, typename G::C1>(t, 1);
+  put_grouped<OPA, OPB, OPC, MODE, typename G::C2>(t, 2);
   using C = ModeCfg<MODE>;
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, false>(t, V_P1);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, false>(t, V_P0);

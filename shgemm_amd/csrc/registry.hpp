@@ -29,6 +29,9 @@ enum : int {
 inline bool variant_is_split(int v) { return v == V_SK || v == V_P1SK || v == V_P0SK; }
 inline int variant_level(int v) { return (v == V_P0 || v == V_P0SK) ? LEVEL_P0 : LEVEL_P1; }
 
+// Tile classes of the grouped kernel (small, 128 x 128, the mode's large tile).
+enum : int { kNumGroupClasses = 3 };
+
 struct KernelTable {
   // [op_a][op_b][op_c][mode][variant]
   KernelEntry shgemm[2][2][2][kNumModes][kNumVariants];
@@ -36,6 +39,8 @@ struct KernelTable {
   KernelEntry sgemm[2][2][2];
   // deep-prefetch prescaled fp16 kernel (op_b = n, full k-tiles): [op_a][op_c]
   KernelEntry shgemm_bd[2][2];
+  // persistent grouped kernels: [op_a][op_b][op_c][mode][tile class]
+  KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,
   // timed by the tile sweep and selectable with MTK_SHGEMM_TUNE_P1=<index>.
   static constexpr int kMaxTune = 16;
@@ -60,11 +65,61 @@ void register_tune(KernelTable&);
 void register_shgemm_bd(KernelTable&);
 void register_all(KernelTable&);
 
+// One problem of a group as the device sees it.  p.ws is the problem's own
+// partial buffer (slice-major, m*n floats per slice) when it is k-sliced,
+// nullptr otherwise; p.row_exp points at the problem's rows of the group's
+// exponent buffer in the prescaled mode.
+struct GroupProblem {
+  GemmParams p;
+  int unit_begin;  // first work unit of the problem
+  int pad;
+};
+
+// Arguments of one persistent grouped launch (one tile class).
+struct GroupLaunch {
+  const GroupProblem* probs;
+  const int* unit_begin;  // probs[i].unit_begin, packed for the search
+  int nprobs;
+  int total_units;
+  int* head;  // queue head, zero before the launch
+};
+
+// Per-problem segment of a grouped row-exponent pass (blocks = bx * by).
+struct RowmaxSeg {
+  const float* a;
+  int64_t lda;
+  uint32_t* row_max;
+  int m, k, kchunk, rq, vec, bx;
+  int block_begin, pad;
+};
+
+// Per-problem segment of a grouped split-K reduction (slices = 0: C = beta*C).
+struct ReduceSeg {
+  const float* ws;
+  float* c;
+  int64_t ldc;
+  int64_t elems;
+  int slices, inner;
+  float alpha, beta;
+  int block_begin, pad;
+};
+
+// Launch geometry of the row-max pass of one matrix.
+struct RowmaxGeometry {
+  int rq, kchunk, bx, by;
+};
+RowmaxGeometry rowmax_geometry(int op_a, int m, int k);
+
 // Auxiliary kernels (defined in aux_kernels.hip).
 hipError_t launch_splitk_reduce(const float* ws, int slices, int64_t elems, int inner, float alpha, float beta,
                                 float* c, int64_t ldc, hipStream_t s);
 hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_t ldc, hipStream_t s);
 hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
                                 int* row_exp, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s);
+hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,
+                                 hipStream_t s);
+hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int nseg, int64_t blocks, hipStream_t s);
 
 }  // namespace shgemm_amd

@@ -1,5 +1,5 @@
 // Host runtime of mtk::shgemm on HIP: handle life cycle, argument checks,
-// kernel choice (tile level / split-K) and launch.
+// kernel choice (plan: tile variant and k-slices) and launch.
 //
 // Reference counterpart: src/shgemm.cu:401-690 (create, destroy, stream and
 // level setters, the shgemm dispatcher).
@@ -9,34 +9,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <mutex>
 #include <string>
 
-#include "../../include/shgemm/shgemm.hpp"
-#include "errors.hpp"
-#include "registry.hpp"
+#include "runtime_internal.hpp"
+
+using shgemm_amd::check_hip;
 
 namespace mtk {
 namespace shgemm {
-namespace detail {
-
-struct handle_state {
-  int device = 0;
-  int num_cus = 0;
-  shgemm_amd::KernelTable table;
-  float* ws = nullptr;  // split-K partials
-  size_t ws_bytes = 0;
-  uint32_t* row_max = nullptr;  // prescale scratch
-  int* row_exp = nullptr;
-  size_t row_cap = 0;
-  bool exp_scaling = true;
-  int tune_p1 = -1;
-  bool use_bd = false;  // deep-prefetch kernel where it applies (opt-in: MTK_SHGEMM_BD=1)
-  int forced_variant = -1;  // MTK_SHGEMM_PLAN=<variant>,<slices>
-  int forced_slices = 1;  // MTK_SHGEMM_TUNE_P1: candidate index for fp16 nn (tile sweep)
-};
-
-}  // namespace detail
 
 namespace {
 
@@ -44,26 +26,15 @@ namespace {
 // include/shgemm/shgemm.hpp:90, sized for one 108-SM GPU).
 constexpr size_t kWorkspaceBytes = size_t(64) << 20;
 
-void check_hip(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw shgemm_amd::hip_failure(std::string(what) + ": " + hipGetErrorString(e));
-}
+using shgemm_amd::aligned;
+using shgemm_amd::group_rows;
+using shgemm_amd::set_lds_limit;
 
 const shgemm_amd::KernelTable& kernel_table() {
   static shgemm_amd::KernelTable t;
   static std::once_flag once;
   std::call_once(once, [] { shgemm_amd::register_all(t); });
   return t;
-}
-
-bool aligned(const void* p, size_t bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
-
-// Tile ordering: group this many tile rows so tiles in flight share panels.
-int group_rows(int tiles_m) { return std::min(tiles_m, 8); }
-
-void set_lds_limit(const shgemm_amd::KernelEntry& e) {
-  if (e.lds > 65536)
-    check_hip(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds),
-              "hipFuncSetAttribute");
 }
 
 void launch(const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmParams& p, hipStream_t s) {
@@ -74,29 +45,6 @@ void launch(const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmP
 
 const char* op_name(operation_t op) { return op == op_n ? "N" : "T"; }
 
-struct Shape {
-  size_t m, n, k;
-};
-
-void validate_common(const shgemmHandle_t& handle, operation_t op_a, operation_t op_b, operation_t op_c, size_t m,
-                     size_t n, size_t k, const float* alpha, const void* a, size_t lda, const void* b, size_t ldb,
-                     const float* beta, const float* c, size_t ldc) {
-  if (handle.state == nullptr) throw error("handle not created (call mtk::shgemm::create first)");
-  if ((op_a != op_n && op_a != op_t) || (op_b != op_n && op_b != op_t) || (op_c != op_n && op_c != op_t))
-    throw error("invalid operation_t");
-  if (m > (size_t)INT_MAX || n > (size_t)INT_MAX || k > (size_t)INT_MAX)
-    throw error("m, n and k must each be below 2^31");
-  if (alpha == nullptr || beta == nullptr) throw error("alpha and beta must be host pointers, not null");
-  const size_t a_rows = op_a == op_n ? m : k, b_rows = op_b == op_n ? k : n, c_rows = op_c == op_n ? m : n;
-  if (lda < std::max<size_t>(1, a_rows)) throw error("lda < rows of the stored A");
-  if (ldb < std::max<size_t>(1, b_rows)) throw error("ldb < rows of the stored B");
-  if (ldc < std::max<size_t>(1, c_rows)) throw error("ldc < rows of the stored C");
-  if (m > 0 && n > 0) {
-    if (c == nullptr) throw error("C is null");
-    if (k > 0 && (a == nullptr || b == nullptr)) throw error("A or B is null");
-  }
-}
-
 // A kernel choice for one call: table variant (or an explicit entry) and the
 // number of k-slices.  `level` is what shgemm returns (reference
 // shgemm.cu:609, 683: P0 for the k-sliced path).
@@ -104,8 +52,10 @@ struct Plan {
   int variant = shgemm_amd::V_P1;
   int slices = 1;
   int level = detail::P1;
+  int requested = shgemm_amd::V_P1;  // plan variant as named by the caller / table (0..5)
+  plan_source source = plan_cost_model;
   const shgemm_amd::KernelEntry* entry = nullptr;
-  const char* source = "";
+  const char* note = "";
 };
 
 // Largest useful split: every slice at least 8 k-tiles, all partials inside
@@ -137,30 +87,69 @@ double plan_cost(const shgemm_amd::KernelEntry& e, int variant, int mode, int64_
   return std::max(t_mfma, t_hbm) + (slices > 1 ? 6e-6 : 0.0);
 }
 
+// The deep-prefetch kernel applies to the prescaled fp16 mode with op_b = N,
+// 16-byte-aligned B and whole k-tiles.
+const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode, int oa, int ob, int oc, int64_t k,
+                                        bool vec_a, bool vec_b) {
+  const auto& bd = st->table.shgemm_bd[oa][oc];
+  if (bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b && k % bd.bk == 0 &&
+      (oa == shgemm_amd::OP_N || vec_a))
+    return &bd;
+  return nullptr;
+}
+
+// A plan variant 0..5 with a slice count -> a launchable plan.  Split
+// variants get 2..max_slices slices; when k or the workspace allows no split
+// they run as the unsplit variant of the same tile.
+Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants],
+                  int requested, int slices, int mode, int oa, int ob, int oc, int64_t m, int64_t n, int64_t k,
+                  bool vec_a, bool vec_b) {
+  Plan pl;
+  pl.requested = requested;
+  if (requested == plan_variant_bd) {
+    pl.variant = shgemm_amd::V_P1;
+    pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
+    if (pl.entry != nullptr) pl.note = " deep-prefetch";
+  } else if (requested >= 0 && requested < shgemm_amd::kNumVariants && tab[requested].fn != nullptr) {
+    pl.variant = requested;
+    if (shgemm_amd::variant_is_split(requested)) {
+      const int64_t smax = max_slices(tab[requested], m, n, k, st->ws_bytes);
+      pl.slices = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(2, slices), smax));
+      if (pl.slices == 1) pl.variant = requested == shgemm_amd::V_P0SK ? shgemm_amd::V_P0 : shgemm_amd::V_P1;
+    }
+  }
+  pl.level = pl.slices > 1 ? detail::P0 : shgemm_amd::variant_level(pl.variant);
+  return pl;
+}
+
 Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
                  const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants], int mode, int oa, int ob, int oc,
                  int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b) {
-  Plan best;
-  const int ncu = std::max(1, st->num_cus);
-  // 1. explicit override (measurement and tuning): MTK_SHGEMM_PLAN=<variant>,<slices>
-  if (st->forced_variant >= 0 && tab[st->forced_variant].fn != nullptr) {
-    best.variant = st->forced_variant;
-    best.slices = shgemm_amd::variant_is_split(best.variant)
-                      ? (int)std::min<int64_t>(st->forced_slices, max_slices(tab[best.variant], m, n, k, st->ws_bytes))
-                      : 1;
-    if (best.slices < 1) best.slices = 1;
-    best.level = best.slices > 1 ? detail::P0 : shgemm_amd::variant_level(best.variant);
-    best.source = " (forced)";
-    return best;
+  // 1. explicit override (measurement and tuning)
+  if (st->forced_variant >= 0) {
+    Plan pl = resolve_plan(st, tab, st->forced_variant, st->forced_slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
+    pl.source = plan_forced;
+    return pl;
   }
   // 2. kernel level fixed by the caller (reference shgemm.cu:655): no split-K
   if (handle.fixed_lernel_level < detail::num_levels) {
-    best.variant = handle.fixed_lernel_level == detail::P1 ? shgemm_amd::V_P1 : shgemm_amd::V_P0;
-    best.level = handle.fixed_lernel_level;
-    best.source = " (fixed level)";
-    return best;
+    Plan pl;
+    pl.variant = handle.fixed_lernel_level == detail::P1 ? shgemm_amd::V_P1 : shgemm_amd::V_P0;
+    pl.requested = pl.variant;
+    pl.level = handle.fixed_lernel_level;
+    pl.source = plan_fixed_level;
+    return pl;
   }
-  // 3. cost model over every variant and split count
+  // 3. measured plan table
+  if (const shgemm_amd::PlanRow* r =
+          shgemm_amd::find_plan(st->plans, shgemm_amd::PLAN_SINGLE, mode, oa, ob, oc, m, n, k)) {
+    Plan pl = resolve_plan(st, tab, r->choice, r->slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
+    pl.source = plan_table;
+    return pl;
+  }
+  // 4. cost model over every variant and split count
+  Plan best;
+  const int ncu = std::max(1, st->num_cus);
   double best_t = 1e30;
   for (int v = 0; v < shgemm_amd::kNumVariants; ++v) {
     const auto& e = tab[v];
@@ -176,23 +165,39 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
       }
     }
   }
+  best.requested = best.variant;
   best.level = best.slices > 1 ? detail::P0 : shgemm_amd::variant_level(best.variant);
+  best.source = plan_cost_model;
   // tile-sweep candidate or deep-prefetch kernel for the large-tile fp16 path
   if (best.slices == 1 && best.variant == shgemm_amd::V_P1) {
     if (st->tune_p1 >= 0 && st->table.tune[st->tune_p1].mode == mode && oa == shgemm_amd::OP_N &&
         ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N) {
       best.entry = &st->table.tune[st->tune_p1];
-      best.source = " (tile sweep candidate)";
-    } else {
-      const auto& bd = st->table.shgemm_bd[oa][oc];
-      if (st->use_bd && bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b &&
-          k % bd.bk == 0 && (oa == shgemm_amd::OP_N || vec_a)) {
-        best.entry = &bd;
-        best.source = " (deep prefetch)";
+      best.note = " tile-sweep candidate";
+    } else if (st->use_bd) {
+      best.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
+      if (best.entry != nullptr) {
+        best.requested = plan_variant_bd;
+        best.note = " deep-prefetch";
       }
     }
   }
   return best;
+}
+
+const char* source_name(plan_source s) {
+  switch (s) {
+    case plan_table: return "table";
+    case plan_forced: return "forced";
+    case plan_fixed_level: return "fixed level";
+    default: return "cost model";
+  }
+}
+
+int compute_mode(const detail::handle_state* st, tc_t compute_type) {
+  return compute_type == tf32 ? shgemm_amd::MODE_BF16
+         : st->exp_scaling    ? shgemm_amd::MODE_FP16S
+                              : shgemm_amd::MODE_FP16;
 }
 
 }  // namespace
@@ -209,6 +214,16 @@ void create(shgemmHandle_t& handle) {
 
   auto* st = new detail::handle_state();
   try {
+    // Measured plan table: $MTK_SHGEMM_PLANS ("none" for no table), else the
+    // table shipped beside the library when it exists.
+    const char* pe = std::getenv("MTK_SHGEMM_PLANS");
+    std::string path = pe != nullptr ? std::string(pe) : shgemm_amd::default_plan_path();
+    if (pe == nullptr && !path.empty() && !std::ifstream(path).good()) path.clear();
+    if (!path.empty() && path != "none") {
+      std::string err;
+      if (!shgemm_amd::parse_plan_file(path, st->plans, err)) throw error("plan table: " + err);
+      st->plans_path = path;
+    }
     check_hip(hipGetDevice(&st->device), "hipGetDevice");
     check_hip(hipDeviceGetAttribute(&st->num_cus, hipDeviceAttributeMultiprocessorCount, st->device),
               "hipDeviceGetAttribute");
@@ -235,16 +250,16 @@ void create(shgemmHandle_t& handle) {
   for (auto& a : st->table.shgemm_bd)
     for (auto& e : a)
       if (e.fn) set_lds_limit(e);
-  // Opt-in: at 176 VGPRs it runs 2 wavefronts per SIMD and measured slower
-  // than the default large tile at 16384^3 (389 vs 473 TFLOP/s effective).
   const char* fp = std::getenv("MTK_SHGEMM_PLAN");
   if (fp != nullptr) {
     int v = -1, sl = 1;
-    if (std::sscanf(fp, "%d,%d", &v, &sl) >= 1 && v >= 0 && v < shgemm_amd::kNumVariants) {
+    if (std::sscanf(fp, "%d,%d", &v, &sl) >= 1 && v >= 0 && v < num_plan_variants) {
       st->forced_variant = v;
       st->forced_slices = std::max(1, sl);
     }
   }
+  // Deep-prefetch kernel where the cost model picks the large tile: opt-in (at
+  // 176 VGPRs it runs 2 wavefronts per SIMD; the tuner measures it per shape).
   const char* bd = std::getenv("MTK_SHGEMM_BD");
   st->use_bd = (bd != nullptr && bd[0] == '1');
   // Per-row exponent prescaling of the fp16 compute type is on unless
@@ -282,12 +297,51 @@ void set_exponent_scaling(shgemmHandle_t& handle, const bool on) {
   handle.state->exp_scaling = on;
 }
 
+void force_plan(shgemmHandle_t& handle, const int variant, const int slices) {
+  if (handle.state == nullptr) throw error("handle not created");
+  if (variant >= num_plan_variants) throw error("plan variant must be below " + std::to_string(num_plan_variants));
+  if (variant >= 0 && (slices < 1 || slices > 256)) throw error("slices must be in 1..256");
+  handle.state->forced_variant = variant < 0 ? -1 : variant;
+  handle.state->forced_slices = variant < 0 ? 1 : slices;
+}
+
+void load_plans(shgemmHandle_t& handle, const char* path) {
+  if (handle.state == nullptr) throw error("handle not created");
+  std::vector<shgemm_amd::PlanRow> rows;
+  if (path != nullptr && path[0] != '\0') {
+    std::string err;
+    if (!shgemm_amd::parse_plan_file(path, rows, err)) throw error("plan table: " + err);
+  }
+  handle.state->plans.swap(rows);
+  handle.state->plans_path = path != nullptr ? path : "";
+}
+
+plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b, const std::size_t m,
+                  const std::size_t n, const std::size_t k, const tc_t compute_type, const operation_t op_c) {
+  if (handle.state == nullptr) throw error("handle not created");
+  if ((op_a != op_n && op_a != op_t) || (op_b != op_n && op_b != op_t) || (op_c != op_n && op_c != op_t))
+    throw error("invalid operation_t");
+  if (compute_type != fp16 && compute_type != tf32) throw error("invalid tc_t");
+  const detail::handle_state* st = handle.state;
+  const int mode = compute_mode(st, compute_type);
+  const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  const Plan pl = choose_plan(handle, st, st->table.shgemm[oa][ob][oc][mode], mode, oa, ob, oc, (int64_t)m, (int64_t)n,
+                              (int64_t)k, true, true);
+  plan_t out;
+  out.variant = pl.entry != nullptr && pl.requested == plan_variant_bd ? plan_variant_bd : pl.variant;
+  out.slices = pl.slices;
+  out.source = pl.source;
+  return out;
+}
+
 detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
                             const std::size_t m, const std::size_t n, const std::size_t k, const float* const alpha_ptr,
                             const float* const a_ptr, const std::size_t lda, const half* const b_ptr,
                             const std::size_t ldb, const float* const beta_ptr, float* const c_ptr,
                             const std::size_t ldc, const tc_t compute_type, const operation_t op_c) {
-  validate_common(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
+  shgemm_amd::validate_args(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
   if (compute_type != fp16 && compute_type != tf32) throw error("invalid tc_t");
   detail::handle_state* st = handle.state;
   const hipStream_t stream = handle.cuda_stream;
@@ -302,9 +356,7 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
     return detail::P0;
   }
 
-  const int mode = compute_type == tf32 ? shgemm_amd::MODE_BF16
-                   : st->exp_scaling ? shgemm_amd::MODE_FP16S
-                                     : shgemm_amd::MODE_FP16;
+  const int mode = compute_mode(st, compute_type);
   const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
@@ -323,8 +375,8 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   p.c = c_ptr;
   p.ldc = (int64_t)ldc;
   // Vector loads: 16 bytes of FP32 for A; 16 (k-contiguous) or 8 bytes of FP16 for B.
-  p.vec_a = aligned(a_ptr, 16) && (lda % 4 == 0);
-  p.vec_b = (ob == shgemm_amd::OP_N) ? (aligned(b_ptr, 16) && ldb % 8 == 0) : (aligned(b_ptr, 8) && ldb % 4 == 0);
+  p.vec_a = shgemm_amd::vec_a_ok(a_ptr, lda);
+  p.vec_b = shgemm_amd::vec_b_ok(b_ptr, ldb, ob);
 
   if (scaled) {
     if (st->row_cap < m) {
@@ -354,9 +406,10 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
   if (tiles > INT_MAX) throw error("too many output tiles");
   if (handle.debug_mode)
-    std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d slices=%d kernel_level=%d%s%s\n", m,
-                n, k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk,
-                plan.slices, plan.level, scaled ? " prescaled" : "", plan.source);
+    std::printf(
+        "[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d slices=%d kernel_level=%d%s plan=%s%s\n",
+        m, n, k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk, plan.slices,
+        plan.level, scaled ? " prescaled" : "", source_name(plan.source), plan.note);
   if (plan.slices > 1) {
     const int64_t kslice = (((int64_t)k + plan.slices - 1) / plan.slices + e.bk - 1) / e.bk * e.bk;
     const int64_t used = ((int64_t)k + kslice - 1) / kslice;
@@ -376,7 +429,7 @@ void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const opera
                 const std::size_t n, const std::size_t k, const float* const alpha_ptr, const float* const a_ptr,
                 const std::size_t lda, const float* const b_ptr, const std::size_t ldb, const float* const beta_ptr,
                 float* const c_ptr, const std::size_t ldc, const operation_t op_c) {
-  validate_common(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
+  shgemm_amd::validate_args(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
   detail::handle_state* st = handle.state;
   const hipStream_t stream = handle.cuda_stream;
   if (m == 0 || n == 0) return;
@@ -420,6 +473,36 @@ void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const opera
 }  // namespace mtk
 
 namespace shgemm_amd {
+
+void set_lds_limit(const KernelEntry& e) {
+  if (e.fn != nullptr && e.lds > 65536)
+    check_hip(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds),
+              "hipFuncSetAttribute");
+}
+
+void validate_args(const mtk::shgemm::shgemmHandle_t& handle, mtk::shgemm::operation_t op_a,
+                   mtk::shgemm::operation_t op_b, mtk::shgemm::operation_t op_c, size_t m, size_t n, size_t k,
+                   const float* alpha, const void* a, size_t lda, const void* b, size_t ldb, const float* beta,
+                   const float* c, size_t ldc) {
+  using mtk::shgemm::error;
+  using mtk::shgemm::op_n;
+  using mtk::shgemm::op_t;
+  if (handle.state == nullptr) throw error("handle not created (call mtk::shgemm::create first)");
+  if ((op_a != op_n && op_a != op_t) || (op_b != op_n && op_b != op_t) || (op_c != op_n && op_c != op_t))
+    throw error("invalid operation_t");
+  if (m > (size_t)INT_MAX || n > (size_t)INT_MAX || k > (size_t)INT_MAX)
+    throw error("m, n and k must each be below 2^31");
+  if (alpha == nullptr || beta == nullptr) throw error("alpha and beta must be host pointers, not null");
+  const size_t a_rows = op_a == op_n ? m : k, b_rows = op_b == op_n ? k : n, c_rows = op_c == op_n ? m : n;
+  if (lda < std::max<size_t>(1, a_rows)) throw error("lda < rows of the stored A");
+  if (ldb < std::max<size_t>(1, b_rows)) throw error("ldb < rows of the stored B");
+  if (ldc < std::max<size_t>(1, c_rows)) throw error("ldc < rows of the stored C");
+  if (m > 0 && n > 0) {
+    if (c == nullptr) throw error("C is null");
+    if (k > 0 && (a == nullptr || b == nullptr)) throw error("A or B is null");
+  }
+}
+
 void register_all(KernelTable& t) {
   register_shgemm_fp16_nn(t);
   register_shgemm_fp16_nt(t);

@@ -1,0 +1,66 @@
+// Host-side internals shared by the runtime units (runtime.cpp, plans.cpp,
+// group.cpp): the state behind a handle and small launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/shgemm/shgemm.hpp"
+#include "errors.hpp"
+#include "plans.hpp"
+#include "registry.hpp"
+
+namespace mtk {
+namespace shgemm {
+namespace detail {
+
+struct handle_state {
+  int device = 0;
+  int num_cus = 0;
+  shgemm_amd::KernelTable table;
+  float* ws = nullptr;  // split-K partials
+  size_t ws_bytes = 0;
+  uint32_t* row_max = nullptr;  // prescale scratch
+  int* row_exp = nullptr;
+  size_t row_cap = 0;
+  bool exp_scaling = true;
+  int tune_p1 = -1;         // MTK_SHGEMM_TUNE_P1: tile-sweep candidate for fp16 nn
+  bool use_bd = false;      // MTK_SHGEMM_BD=1: deep-prefetch kernel where the cost model picks P1
+  int forced_variant = -1;  // force_plan / MTK_SHGEMM_PLAN=<variant>,<slices>
+  int forced_slices = 1;
+  std::vector<shgemm_amd::PlanRow> plans;  // measured plan table
+  std::string plans_path;
+};
+
+}  // namespace detail
+}  // namespace shgemm
+}  // namespace mtk
+
+namespace shgemm_amd {
+
+inline void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw hip_failure(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline bool aligned(const void* p, size_t bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
+
+// Tile ordering: group this many tile rows so tiles in flight share panels.
+inline int group_rows(int tiles_m) { return tiles_m < 8 ? tiles_m : 8; }
+
+void set_lds_limit(const KernelEntry& e);
+
+// Argument checks shared by shgemm, sgemm_fp32 and the grouped entry points
+// (throws mtk::shgemm::error).
+void validate_args(const mtk::shgemm::shgemmHandle_t& handle, mtk::shgemm::operation_t op_a,
+                   mtk::shgemm::operation_t op_b, mtk::shgemm::operation_t op_c, size_t m, size_t n, size_t k,
+                   const float* alpha, const void* a, size_t lda, const void* b, size_t ldb, const float* beta,
+                   const float* c, size_t ldc);
+
+// Vector-load flags of an SHGEMM operand pair (16-byte FP32 A; 16- or 8-byte FP16 B).
+inline int vec_a_ok(const float* a, size_t lda) { return aligned(a, 16) && lda % 4 == 0; }
+inline int vec_b_ok(const void* b, size_t ldb, int op_b) {
+  return op_b == OP_N ? (aligned(b, 16) && ldb % 8 == 0) : (aligned(b, 8) && ldb % 4 == 0);
+}
+
+}  // namespace shgemm_amd

@@ -485,9 +485,11 @@ __device__ __forceinline__ void mma_ktile(f32x16 (&acc)[NACC][FM][FN], const uin
 
 // Epilogue.  Lane/register -> (m, n) of the 32x32 MFMA output block: the lane
 // index gives the column, register e gives row (e/4)*8 + (lane/32)*4 + e%4.
-template <int MODE, bool LANE_M, bool SPLITK, int NACC, int FM, int FN, int WM, int WN>
+// TO_WS: the tile is one k-slice's partial; it goes unscaled by alpha/beta to
+// `part` (m*n floats in C's storage order, leading dimension m or n).
+template <int MODE, bool LANE_M, bool TO_WS, int NACC, int FM, int FN, int WM, int WN>
 __device__ __forceinline__ void store_ctile(const f32x16 (&acc)[NACC][FM][FN], const GemmParams& p, int m0, int n0,
-                                            int wm, int wn, int lrow, int lh) {
+                                            int wm, int wn, int lrow, int lh, float* __restrict__ part) {
   constexpr bool SCALED = (MODE == MODE_FP16S);
   constexpr float lo_w = 1.0f / 2048.0f;
   const float alpha = p.alpha, beta = p.beta;
@@ -510,10 +512,8 @@ __device__ __forceinline__ void store_ctile(const f32x16 (&acc)[NACC][FM][FN], c
           float v = acc[0][i][j][e];
           if constexpr (NACC == 2) v = __builtin_fmaf(acc[1][i][j][e], lo_w, v);
           if constexpr (SCALED) v = __builtin_ldexpf(v, p.row_exp[mm]);
-          if constexpr (SPLITK) {
-            const int64_t off = (int64_t)blockIdx.y * p.m * p.n +
-                                (LANE_M ? (mm + (int64_t)nn * p.m) : (nn + (int64_t)mm * p.n));
-            p.ws[off] = v;
+          if constexpr (TO_WS) {
+            part[LANE_M ? (mm + (int64_t)nn * p.m) : (nn + (int64_t)mm * p.n)] = v;
           } else {
             float* cp = p.c + (LANE_M ? (mm + nn * p.ldc) : (nn + mm * p.ldc));
             *cp = (beta == 0.0f) ? alpha * v : alpha * v + beta * *cp;
@@ -525,13 +525,16 @@ __device__ __forceinline__ void store_ctile(const f32x16 (&acc)[NACC][FM][FN], c
 }
 
 // ---------------------------------------------------------------------------
-// The kernel.
+// One output tile (tm, tn) over k in [kbeg, kend) by one workgroup.
 //   OPA, OPB: operand storage (OP_N / OP_T), OPC: C storage.
 //   MODE: split scheme (MODE_FP16S prescales A rows by 2^-row_exp).
-//   SPLITK: blockIdx.y selects a k-slice; the partial goes to p.ws.
+//   SPLIT: 0 writes C (alpha, beta applied); 1 writes the k-slice partial to
+//   `part`; 2 decides at run time (part != nullptr).
+// Leaves LDS free for the next tile: the last k-tile ends with a barrier.
 // ---------------------------------------------------------------------------
-template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
-__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmParams p) {
+template <int OPA, int OPB, int OPC, int MODE, class CFG, int SPLIT>
+__device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn, int kbeg, int kend,
+                                            float* __restrict__ part, uint4* __restrict__ lds) {
   constexpr bool SCALED = (MODE == MODE_FP16S);
   constexpr int NACC = (MODE == MODE_FP16) ? 2 : 1;  // separate hi / lo accumulators
   constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC;
@@ -542,7 +545,6 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
   // With C column-major the MFMA's lane index runs along m (coalesced C
   // stores); with C transposed it runs along n.
   constexpr bool LANE_M = (OPC == OP_N);
-  extern __shared__ uint4 lds[];
 
   constexpr int PF = CFG::PF;
   using S0 = std::integral_constant<int, 0>;
@@ -554,14 +556,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
-  int tm, tn;
-  tile_coords(p, blockIdx.x, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  int kbeg = 0, kend = p.k;
-  if constexpr (SPLITK) {
-    kbeg = blockIdx.y * p.k_slice;
-    kend = min(p.k, kbeg + p.k_slice);
-  }
   const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const float* __restrict__ A = p.a;
   const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
@@ -646,7 +641,29 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
     }
   }
 
-  store_ctile<MODE, LANE_M, SPLITK, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh);
+  if constexpr (SPLIT == 2) {
+    if (part != nullptr) store_ctile<MODE, LANE_M, true, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+    else store_ctile<MODE, LANE_M, false, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+  } else {
+    store_ctile<MODE, LANE_M, SPLIT == 1, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+  }
+}
+
+// The single-problem kernel: blockIdx.x -> output tile (XCD-aware order);
+// with SPLITK, blockIdx.y selects the k-slice and the partial goes to
+// p.ws + slice * m * n.
+template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmParams p) {
+  extern __shared__ uint4 lds[];
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  if constexpr (SPLITK) {
+    const int kbeg = blockIdx.y * p.k_slice;
+    shgemm_tile<OPA, OPB, OPC, MODE, CFG, 1>(p, tm, tn, kbeg, min(p.k, kbeg + p.k_slice),
+                                             p.ws + (int64_t)blockIdx.y * p.m * p.n, lds);
+  } else {
+    shgemm_tile<OPA, OPB, OPC, MODE, CFG, 0>(p, tm, tn, 0, p.k, nullptr, lds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -752,7 +769,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_bd(const Gemm
     publish(kt + 3 < ktiles);
   }
 
-  store_ctile<MODE, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh);
+  store_ctile<MODE, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
 }
 
 }  // namespace shgemm_amd

@@ -1,0 +1,222 @@
+"""Autotuner: measures candidate kernel plans on the GPU and writes the plan
+table the runtime loads (plans_gfx950.tsv beside libshgemm_amd.so).
+
+Replaces the reference's static per-architecture tile tables and its
+level / split-K heuristic (src/shgemm.cu:427-489, 532, 643-656) with
+measurements on the device at hand.
+
+Two kinds of rows (format in csrc/plans.hpp):
+  single   one shgemm call at shape point (2^lm, 2^ln, 2^lk): every plan
+           variant (P0, P1, P1 three-stage, and the split-K variants at 2..256
+           slices) is timed alone; the fastest is kept.
+  grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
+           2 large) x k-slices {1, 2, 4} are timed on a group of identical
+           copies large enough to fill the GPU several times over, so the
+           choice is the best throughput inside a busy group.
+
+Usage (on a GPU):
+  python -m shgemm_amd.tune --out shgemm_amd/plans_gfx950.tsv
+  python -m shgemm_amd.tune --quick --out /tmp/plans.tsv     # reduced grid
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import math
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import shgemm_amd as sg
+
+MODES = {  # plan-table mode -> (compute type, exponent scaling)
+    "fp16s": (sg.fp16, True),
+    "fp16": (sg.fp16, False),
+    "bf16": (sg.tf32, True),
+}
+OPS = {"N": sg.op_n, "T": sg.op_t}
+
+SINGLE_M = [128, 512, 2048, 8192, 16384]
+SINGLE_N = [64, 256, 1024, 4096, 16384]
+SINGLE_K = [256, 1024, 4096, 16384, 65536, 262144, 1048576]
+QUICK_M = [128, 2048, 16384]
+QUICK_N = [64, 1024, 16384]
+QUICK_K = [1024, 16384, 1048576]
+MAX_BYTES = 2.5e9       # operands + C of one tuning shape
+MAX_FLOPS = 2.0 ** 41   # 2mnk of one tuning shape
+
+GROUP_M = [256, 1024, 4096, 8192]
+GROUP_N = [64, 128, 256, 512]
+GROUP_K = [256, 1024, 4096, 16384]
+QUICK_GROUP_M = [256, 4096]
+QUICK_GROUP_N = [64, 512]
+QUICK_GROUP_K = [1024, 16384]
+GROUP_SLICES = [1, 2, 4]
+GROUP_UNITS = 8192      # work units per timed group (about 16 per workgroup slot)
+GROUP_MAX_COPIES = 64
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise SystemExit("tune.py needs a GPU")
+    return torch
+
+
+def time_fn(fn, min_ms: float = 2.0, reps: int = 3) -> float:
+    """Median per-call milliseconds of fn over `reps` timed batches, each batch
+    long enough (>= min_ms) to hide launch jitter."""
+    torch = _torch()
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    fn()
+    e.record()
+    torch.cuda.synchronize()
+    one = max(s.elapsed_time(e), 1e-3)
+    iters = max(1, min(200, int(math.ceil(min_ms / one))))
+    out = []
+    for _ in range(reps):
+        s.record()
+        for _ in range(iters):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        out.append(s.elapsed_time(e) / iters)
+    return statistics.median(out)
+
+
+def single_shapes(quick: bool):
+    ms, ns, ks = (QUICK_M, QUICK_N, QUICK_K) if quick else (SINGLE_M, SINGLE_N, SINGLE_K)
+    for m in ms:
+        for n in ns:
+            for k in ks:
+                if 4.0 * m * k + 2.0 * k * n + 4.0 * m * n > MAX_BYTES or 2.0 * m * n * k > MAX_FLOPS:
+                    continue
+                yield m, n, k
+
+
+def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
+    torch = _torch()
+    ct, scaling = MODES[mode]
+    oa, ob, oc = (OPS[c] for c in ops)
+    sg.set_exponent_scaling(h, scaling)
+    rows = []
+    for m, n, k in single_shapes(quick):
+        a_rows, b_rows, c_rows = (m if oa == sg.op_n else k), (k if ob == sg.op_n else n), (m if oc == sg.op_n else n)
+        A = torch.rand(m * k, device="cuda") * 2 - 1
+        B = torch.randn(k * n, device="cuda").half()
+        C = torch.empty(m * n, device="cuda")
+        seen, best = set(), None
+        candidates = [(0, 1), (1, 1), (5, 1)] + [(v, s) for v in (2, 3, 4) for s in (2, 4, 8, 16, 32, 64, 128, 256)]
+        for v, s in candidates:
+            sg.force_plan(h, v, s)
+            eff = sg.query_plan(h, oa, ob, m, n, k, ct, oc)[:2]
+            if eff in seen:
+                continue
+            seen.add(eff)
+            call = lambda: sg.shgemm(h, oa, ob, m, n, k, 1.0, A, a_rows, B, b_rows, 0.0, C, c_rows, ct, oc)
+            t = time_fn(call)
+            if best is None or t < best[0]:
+                best = (t, eff)
+        sg.force_plan(h, -1, 1)
+        t, (v, s) = best
+        lm, ln, lk = (int(round(math.log2(x))) for x in (m, n, k))
+        rows.append(f"single\t{mode}\t{ops[0]}\t{ops[1]}\t{ops[2]}\t{lm}\t{ln}\t{lk}\t{v}\t{s}\t{t * 1e3:.1f}")
+        log(f"single {mode} {ops} m={m} n={n} k={k}: {sg.PLAN_VARIANTS[v]} x{s} {t * 1e3:.1f} us "
+            f"({2.0 * m * n * k / t / 1e9:.1f} TFLOP/s, {len(seen)} plans)")
+        del A, B, C
+        torch.cuda.empty_cache()
+    return rows
+
+
+def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
+    torch = _torch()
+    ct, scaling = MODES[mode]
+    sg.set_exponent_scaling(h, scaling)
+    gm, gn, gk = (QUICK_GROUP_M, QUICK_GROUP_N, QUICK_GROUP_K) if quick else (GROUP_M, GROUP_N, GROUP_K)
+    rows = []
+    for m in gm:
+        for n in gn:
+            for k in gk:
+                A = torch.rand(m * k, device="cuda") * 2 - 1
+                B = torch.randn(k * n, device="cuda").half()
+                best = None
+                for cls in (0, 1, 2):
+                    for s in GROUP_SLICES:
+                        # copies so the group has about GROUP_UNITS work units
+                        tiles = _tiles(cls, mode, m, n)
+                        copies = max(1, min(GROUP_MAX_COPIES, GROUP_UNITS // max(1, tiles * s)))
+                        C = torch.empty(copies * m * n, device="cuda")
+                        probs = [(m, n, k, A, m, B, k, C[i * m * n:(i + 1) * m * n], m, 1.0, 0.0)
+                                 for i in range(copies)]
+                        with sg.Group(h, sg.op_n, sg.op_n, probs, ct, tile_class=[cls] * copies,
+                                      slices=[s] * copies) as grp:
+                            _, used, _ = grp.plan()
+                            if used[0] != s:
+                                continue  # k too short for this many slices
+                            t = time_fn(lambda: grp.run()) / copies
+                        del C
+                        if best is None or t < best[0]:
+                            best = (t, cls, s)
+                t, cls, s = best
+                lm, ln, lk = (int(round(math.log2(x))) for x in (m, n, k))
+                rows.append(f"grouped\t{mode}\tN\tN\tN\t{lm}\t{ln}\t{lk}\t{cls}\t{s}\t{t * 1e3:.2f}")
+                log(f"grouped {mode} m={m} n={n} k={k}: class {cls} x{s} {t * 1e3:.2f} us/problem "
+                    f"({2.0 * m * n * k / t / 1e9:.1f} TFLOP/s)")
+                del A, B
+                torch.cuda.empty_cache()
+    return rows
+
+
+# Tile shapes of the grouped classes (csrc/grouped_kernel.hpp GroupCfg; the
+# prescaled mode's large tile is 128 x 256, the others' 128 x 128).
+def _tiles(cls: int, mode: str, m: int, n: int) -> int:
+    bm, bn = {0: (64, 64), 1: (128, 128), 2: (128, 256 if mode == "fp16s" else 128)}[cls]
+    return -(-m // bm) * -(-n // bn)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--quick", action="store_true", help="reduced shape grid")
+    ap.add_argument("--single", default="fp16s:NNN,fp16s:TNN,fp16s:TTT,fp16:NNN,bf16:NNN",
+                    help="mode:ops list for single-call rows ('' for none)")
+    ap.add_argument("--grouped", default="fp16s,bf16", help="modes for grouped rows ('' for none)")
+    args = ap.parse_args(argv)
+    torch = _torch()
+    t0 = time.time()
+    log = lambda msg: print(f"[tune {time.time() - t0:6.1f}s] {msg}", flush=True)
+    h = sg.create()
+    sg.load_plans(h, None)  # measure from the cost model's state, not an older table
+    rows = []
+    try:
+        for item in filter(None, args.single.split(",")):
+            mode, ops = item.split(":")
+            rows += tune_single(h, mode, ops, args.quick, log)
+        for mode in filter(None, args.grouped.split(",")):
+            rows += tune_grouped(h, mode, args.quick, log)
+    finally:
+        sg.destroy(h)
+    props = torch.cuda.get_device_properties(0)
+    head = [
+        "# shgemm_amd plan table (kind mode op_a op_b op_c log2m log2n log2k choice slices time_us)",
+        f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}, {props.multi_processor_count} CUs)",
+        f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
+        f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
+        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P1 three-stage);"
+        " time_us per call",
+        "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large); time_us per problem in a full group",
+    ]
+    out = Path(args.out)
+    out.write_text("\n".join(head + rows) + "\n")
+    n = sg.check_plan_file(out)
+    log(f"wrote {out} ({n} rows)")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

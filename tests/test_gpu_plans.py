@@ -1,0 +1,114 @@
+"""GPU: kernel-plan selection (forced plan, fixed level, measured plan table,
+cost model) and that a plan from the table runs the kernel it names."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_util import dev_buffer, read_buffer, require_gpu
+
+pytestmark = pytest.mark.gpu
+COST, TABLE, FORCED, FIXED = 0, 1, 2, 3
+
+
+@pytest.fixture()
+def h():
+    require_gpu()
+    import shgemm_amd as sg
+
+    handle = sg.create()
+    yield handle
+    sg.destroy(handle)
+
+
+def test_plan_sources_and_precedence(h, tmp_path):
+    import shgemm_amd as sg
+
+    table = tmp_path / "plans.tsv"
+    table.write_text("# test table\n"
+                     "single fp16s N N N 10 10 10 0 1\n"
+                     "single fp16s N N N 7 7 16 2 4 12.5\n"
+                     "single bf16 N N N 10 10 10 1 1\n")
+    sg.load_plans(h, table)
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16) == (0, 1, TABLE)
+    assert sg.query_plan(h, 0, 0, 1100, 900, 1000, sg.fp16) == (0, 1, TABLE)   # nearest point
+    assert sg.query_plan(h, 0, 0, 128, 128, 1 << 16, sg.fp16) == (2, 4, TABLE)
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.tf32) == (1, 1, TABLE)
+    # layout not in the table: falls back to the N/N/N rows of the mode
+    assert sg.query_plan(h, 1, 1, 1024, 1024, 1024, sg.fp16, 1) == (0, 1, TABLE)
+    # far from every point: cost model
+    assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[2] == COST
+    # forced beats the table, fixed level beats the table, forced beats fixed
+    sg.enable_kernel_level_fixing(h, 1)
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16) == (1, 1, FIXED)
+    sg.force_plan(h, 4, 8)
+    v, s, src = sg.query_plan(h, 0, 0, 1024, 1024, 1 << 16, sg.fp16)
+    assert (v, s, src) == (4, 8, FORCED)
+    sg.force_plan(h, -1, 1)
+    sg.disable_kernel_level_fixing(h)
+    sg.load_plans(h, None)
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[2] == COST
+
+
+def test_forced_split_is_clamped(h):
+    import shgemm_amd as sg
+
+    sg.force_plan(h, 2, 256)
+    v, s, src = sg.query_plan(h, 0, 0, 128, 128, 4096, sg.fp16)
+    assert src == FORCED and v == 2 and 2 <= s <= 16  # every slice keeps >= 8 k-tiles of 32
+    # a split variant with k too short for two slices runs unsplit
+    v, s, _ = sg.query_plan(h, 0, 0, 128, 128, 300, sg.fp16)
+    assert s == 1 and v == 1
+    sg.force_plan(h, -1, 1)
+
+
+def test_bad_plan_table_is_rejected(h, tmp_path):
+    import shgemm_amd as sg
+
+    bad = tmp_path / "bad.tsv"
+    bad.write_text("single fp16s N N N 10 10 10 9 1\n")  # variant 9 does not exist
+    with pytest.raises(sg.ShgemmError) as ei:
+        sg.load_plans(h, bad)
+    assert ei.value.status == sg.STATUS_EINVAL and "bad.tsv:1" in str(ei.value)
+    with pytest.raises(sg.ShgemmError):
+        sg.force_plan(h, 6, 1)
+
+
+def test_table_plan_runs_named_kernel(h, tmp_path):
+    """A table row selecting split-K gives the same bits as forcing that plan,
+    and a different row (unsplit) stays within the error bar."""
+    import shgemm_amd as sg
+
+    m, n, k = 128, 128, 1 << 14
+    rng = np.random.default_rng(11)
+    A = oracle.gen_uniform(rng, m * k)
+    B16 = oracle.gen_f16_normal(rng, k * n)
+    a_t, a_p = dev_buffer(A)
+    b_t, b_p = dev_buffer(B16)
+    c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
+    table = tmp_path / "plans.tsv"
+    table.write_text("single fp16s N N N 7 7 14 2 8\n")
+    sg.load_plans(h, table)
+    assert sg.query_plan(h, 0, 0, m, n, k, sg.fp16) == (2, 8, TABLE)
+    sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, sg.fp16)
+    from_table = read_buffer(c_t, 0, m * n).copy()
+    sg.load_plans(h, None)
+    sg.force_plan(h, 2, 8)
+    sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, sg.fp16)
+    forced = read_buffer(c_t, 0, m * n).copy()
+    sg.force_plan(h, -1, 1)
+    assert np.array_equal(from_table.view(np.uint32), forced.view(np.uint32))
+    ref = A.reshape(k, m).T.astype(np.float64) @ B16.astype(np.float64).reshape(n, k).T
+    err = np.linalg.norm(from_table.reshape(n, m).T - ref) / np.linalg.norm(ref)
+    assert err < 1e-6
+
+
+def test_shipped_table_loads_at_create(h):
+    import shgemm_amd as sg
+
+    shipped = sg.LIB_PATH.parent / "plans_gfx950.tsv"
+    if not shipped.exists():
+        pytest.skip("no shipped plan table")
+    assert sg.check_plan_file(shipped) > 0
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[2] == TABLE

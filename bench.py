@@ -31,6 +31,9 @@ PEAK_HBM = 8.0             # TB/s
 
 def parse():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["square", "grouped", "serve"], default="square",
+                    help="square: m=n=k=--size (the headline line); grouped: 512 uneven problems in one group; "
+                         "serve: sketch server under Poisson arrivals")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -40,6 +43,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device copies (reference for FETCH_SIZE/WRITE_SIZE scaling)")
+    ap.add_argument("--problems", type=int, default=512, help="grouped workload: number of problems")
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--rates", default="200,500,1000,2000", help="serve: offered request rates (req/s)")
+    ap.add_argument("--seconds", type=float, default=3.0, help="serve: arrival window per rate")
+    ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent lanes (handle + stream each)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget")
     return ap.parse_args()
 
 
@@ -102,12 +111,219 @@ def _cpu_out(m, n):
     return _OUT["c"]
 
 
+def grouped_problems(count, seed):
+    """The batched workload: m log-uniform in [256, 8192], n in {64, 128, 256,
+    512}, k log-uniform in [256, 16384] (seeded)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    m = np.exp(rng.uniform(np.log(256), np.log(8192), count)).astype(int)
+    k = np.exp(rng.uniform(np.log(256), np.log(16384), count)).astype(int)
+    n = rng.choice([64, 128, 256, 512], count)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(m, n, k)]
+
+
+def bench_grouped(args):
+    """One step = the whole batch.  Timed three ways on the same operands:
+    the persistent grouped launch (one graph replay), one shgemm launch per
+    problem, and those per-problem launches captured in one graph."""
+    import numpy as np
+    import torch
+
+    import shgemm_amd as sg
+
+    dev = torch.device("cuda:0")
+    shapes = grouped_problems(args.problems, args.seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    As, Bs, Cs = [], [], []
+    for m, n, k in shapes:
+        As.append(torch.rand(m * k, device=dev, generator=g) * 2 - 1)
+        Bs.append((torch.randn(k * n, device=dev, generator=g) / float(np.sqrt(n))).half())
+        Cs.append(torch.empty(m * n, device=dev))
+    probs = [(m, n, k, A, m, B, k, C, m, 1.0, 0.0) for (m, n, k), A, B, C in zip(shapes, As, Bs, Cs)]
+    flops = sum(2.0 * m * n * k for m, n, k in shapes)
+    byts = sum(4.0 * m * k + 2.0 * k * n + 4.0 * m * n for m, n, k in shapes)
+
+    stream = torch.cuda.current_stream()
+    h = sg.create()
+    sg.set_cuda_stream(h, stream)
+    grp = sg.Group(h, sg.op_n, sg.op_n, probs, sg.fp16)
+    cls, slices, units = grp.plan()
+    run_group = lambda: grp.run(stream)
+    g_region, g_wall, g_ms = timed_launches(run_group, args.steps, args.warmup, stream)
+
+    def per_problem():
+        for m, n, k, A, lda, B, ldb, C, ldc, al, be in probs:
+            sg.shgemm(h, sg.op_n, sg.op_n, m, n, k, al, A, lda, B, ldb, be, C, ldc, sg.fp16)
+
+    p_region, _, p_ms = timed_launches(per_problem, max(2, args.steps // 2), 1, stream)
+    # the same per-problem launches replayed from one captured graph
+    pg_ms = None
+    try:
+        cs = torch.cuda.Stream()
+        sg.set_cuda_stream(h, cs)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cs):
+            per_problem()
+        sg.set_cuda_stream(h, stream)
+        _, _, pg_ms = timed_launches(lambda: graph.replay(), max(2, args.steps // 2), 1, stream)
+    except Exception as e:  # capture unsupported: report as not measured
+        print(f"bench.py: per-problem graph capture failed: {e}", file=sys.stderr)
+        sg.set_cuda_stream(h, stream)
+
+    # accuracy: FP64 error of sampled rows of a few problems, beside SGEMM's
+    run_group()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(1)
+    errs = []
+    for i in rng.choice(len(shapes), size=min(8, len(shapes)), replace=False):
+        m, n, k = shapes[i]
+        rows = np.sort(rng.choice(m, size=min(64, m), replace=False))
+        ridx = torch.from_numpy(rows).to(dev)
+        B32 = Bs[i].float()
+        Cs32 = torch.empty(m * n, device=dev)
+        sg.sgemm_fp32(h, sg.op_n, sg.op_n, m, n, k, 1.0, As[i], m, B32, k, 0.0, Cs32, m)
+        torch.cuda.synchronize()
+        A_rows = As[i].view(k, m).t()[ridx].cpu().numpy()
+        Bh = B32.view(n, k).t().cpu().numpy()
+        e = fp64_sample_error(A_rows, Bh, Cs[i].view(n, m).t()[ridx].cpu().numpy())
+        es = fp64_sample_error(A_rows, Bh, Cs32.view(n, m).t()[ridx].cpu().numpy())
+        errs.append(e / es)
+    grp.destroy()
+    sg.destroy(h)
+
+    ms_per_step = g_region / args.steps
+    value = flops / (ms_per_step * 1e-3) / 1e12
+    t_roof = max(flops / (PEAK_FP16_DENSE / 2.0 * 1e12), byts / (PEAK_HBM * 1e12))
+    hist = {c: int(sum(1 for x in cls if x == c)) for c in (0, 1, 2)}
+    line = {
+        "metric": "eff. TFLOP/s, batched uneven SHGEMM (one grouped launch)",
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 MFMA (f32 split hi+lo) / f32 accumulate",
+        "data": "synthetic (seeded torch RNG on device)",
+        "config": {"workload": f"grouped shgemm, {len(shapes)} problems, m logU[256,8192], n in {{64..512}}, "
+                               f"k logU[256,16384], A U[-1,1], B N(0,1/n), op_n/op_n fp16",
+                   "problems": len(shapes), "total_flops": flops, "total_bytes": byts,
+                   "work_units": units, "tile_classes": hist,
+                   "split_problems": int(sum(1 for s in slices if s > 1)), "parallelism": "single GPU"},
+        "comparator": {"per_problem_launch_ms": round(p_region / max(2, args.steps // 2), 4),
+                       "per_problem_launch_tflops": round(flops / (p_region / max(2, args.steps // 2) * 1e-3) / 1e12, 3),
+                       "per_problem_graph_ms": round(pg_ms, 4) if pg_ms else None,
+                       "grouped_speedup_vs_per_problem": round(p_region / max(2, args.steps // 2) / ms_per_step, 3)},
+        "error": {"max_ratio_vs_sgemm": max(errs), "sampled_problems": len(errs), "rows_per_problem": 64},
+        "roofline": {"bound": "mfma" if flops / (PEAK_FP16_DENSE / 2.0 * 1e12) > byts / (PEAK_HBM * 1e12) else "hbm",
+                     "achieved": round(flops / (g_ms * 1e-3) / 1e12, 3), "peak": round(PEAK_FP16_DENSE / 2.0, 1),
+                     "unit": "TFLOP/s", "frac": round(t_roof * 1e3 / g_ms, 4), "traffic": None,
+                     "note": "frac = batch roofline time max(flops/1258 TF, bytes/8 TB/s) / measured graph time"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Sketch serving: Poisson arrivals at each offered rate; per request an
+    FP32 A (m logU[1024,32768] x k logU[1024,16384]) resident in HBM (a block
+    of one of four 32768 x 16384 matrices), l in {64..1024}, Omega generated on
+    the device from the request seed; p50 / p99 latency (arrival to observed
+    completion) and the sustained rate."""
+    import numpy as np
+    import torch
+
+    from shgemm_amd.serve import GpuLanes, SketchRequest, SketchServer, poisson_offsets, run_arrivals, sketch_shapes
+
+    dev = torch.device("cuda:0")
+    ROWS, COLS, NPOOL = 32768, 16384, 4
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    pool = [torch.rand(ROWS * COLS, device=dev, generator=g) * 2 - 1 for _ in range(NPOOL)]
+    rng = np.random.default_rng(args.seed)
+
+    def make(n, rid0):
+        out = []
+        for i, (m, k, l) in enumerate(sketch_shapes(n, rng)):
+            p = (rid0 + i) % NPOOL
+            r0 = int(rng.integers(0, ROWS - m + 1)) // 4 * 4
+            c0 = int(rng.integers(0, COLS - k + 1))
+            addr = pool[p].data_ptr() + 4 * (c0 * ROWS + r0)
+            out.append(SketchRequest(rid=rid0 + i, m=m, k=k, l=l, seed=rid0 + i, a=addr, lda=ROWS))
+        return out
+
+    lanes = GpuLanes(args.lanes)
+    lanes.reserve(ROWS, COLS, 1024)
+    # warm-up: every lane, plan lookups and the allocator
+    srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
+    run_arrivals(srv, make(4 * args.lanes, 10 ** 6), [0.0] * (4 * args.lanes))
+    torch.cuda.synchronize()
+    results = []
+    for rate in [float(x) for x in args.rates.split(",") if x]:
+        n = max(20, int(rate * args.seconds))
+        reqs = make(n, len(results) * 10 ** 5)
+        srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
+        st = run_arrivals(srv, reqs, poisson_offsets(n, rate, rng))
+        st["offered_rate"] = rate
+        st["mean_request_gflop"] = sum(r.flops for r in reqs) / n / 1e9
+        results.append(st)
+        print(f"bench.py serve: rate {rate:.0f}/s -> p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
+              f"sustained {st['sustained_req_s']:.0f}/s peak lanes {st['peak_inflight']}", file=sys.stderr, flush=True)
+        del reqs
+    # saturation: every request present at time zero
+    n = max(40, int(max(r["offered_rate"] for r in results) * args.seconds / 2))
+    reqs = make(n, 9 * 10 ** 5)
+    srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
+    sat = run_arrivals(srv, reqs, [0.0] * n)
+    lanes.close()
+    ok = [r for r in results if r["sustained_req_s"] >= 0.95 * r["offered_rate"]]
+    head = max(ok, key=lambda r: r["offered_rate"]) if ok else results[0]
+    line = {
+        "metric": "sketch p99 latency at sustained req/s (Y = A*Omega, continuous batching)",
+        "value": round(head["p99_ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": int(sum(r["requests"] for r in results)),
+        "warmup": 4 * args.lanes,
+        "ms_per_step": round(head["mean_ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 MFMA (f32 split hi+lo) / f32 accumulate",
+        "data": "synthetic: A resident (seeded torch RNG), Omega from request seed on device",
+        "config": {"workload": "sketch server, Poisson arrivals, m logU[1024,32768], k logU[1024,16384], "
+                               "l in {64,128,256,512,1024}, fp16 compute type",
+                   "lanes": args.lanes, "hbm_budget_gb": args.budget_gb, "seconds_per_rate": args.seconds,
+                   "parallelism": "single GPU"},
+        "sustained_req_s": round(head["sustained_req_s"], 1),
+        "offered_req_s": head["offered_rate"],
+        "p50_ms": round(head["p50_ms"], 3),
+        "per_rate": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
+        "saturation": {"requests": sat["requests"], "req_s": round(sat["sustained_req_s"], 1),
+                       "tflops": round(sat["flops_per_s"] / 1e12, 2)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "grouped":
+        bench_grouped(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
+        return
 
     import numpy as np
     import torch

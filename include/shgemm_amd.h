@@ -123,6 +123,18 @@ int shgemm_amd_group_run(shgemm_amd_group* g, void* stream, int use_graph);
 int shgemm_amd_group_plan(shgemm_amd_group* g, int* tile_class, int* slices, size_t* units);
 int shgemm_amd_group_destroy(shgemm_amd_group* g);
 
+/* ---- Sketch matrices -----------------------------------------------------
+ * No counterpart in the reference (its tests fill inputs with cuRAND Philox,
+ * test/shgemm_test.cu:247-252).  Counter-based Philox4x32-10 (Salmon et al.,
+ * SC'11) with key = seed and counter = (block mod 2^32, block / 2^32, 0, 0);
+ * element i depends only on (seed, i).  Enqueued on `stream` (NULL = default)
+ * on the current device. */
+/* out[i] = FP16(scale * N(0,1)): Box-Muller on word pairs (0,1), (2,3) of
+ * block i/4, with u = ((w >> 9) + 1/2) * 2^-23 in (0, 1). */
+int shgemm_amd_gaussian_f16(uint16_t* out, size_t count, uint64_t seed, float scale, void* stream);
+/* out[4b + j] = word j of block b, b < blocks (out 16-byte aligned). */
+int shgemm_amd_philox_words(uint32_t* out, size_t blocks, uint64_t seed, void* stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* shgemm_amd_last_error(void);
 

@@ -13,6 +13,9 @@ Contents
   emulate_shgemm             FP64 evaluation of the split representation the
                              GPU multiplies: alpha*(op(hi) + 2^-11 op(lo)) op(B) + beta*C0
   rel_fro_error              ||C - C64||_F / ||C64||_F (the error metric)
+  philox4x32_10, philox_words, gaussian
+                             the sketch generator (Philox4x32-10 + Box-Muller),
+                             restated for checking the device Omega
   gen_*                      seeded input generators shared by tests and bench
 """
 from __future__ import annotations
@@ -62,6 +65,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_f32_to_bf16.argtypes = [f32]
         L.oracle_f32_to_bf16.restype = ctypes.c_uint16
         L.oracle_max_threads.restype = i32
+        L.oracle_philox4x32_10.argtypes = [vp, vp, vp]
+        L.oracle_philox_words.argtypes = [vp, i64, ctypes.c_uint64]
+        L.oracle_gaussian.argtypes = [vp, i64, ctypes.c_uint64, f64]
         _lib = L
     return _lib
 
@@ -290,3 +296,26 @@ def gen_wide_exponent(rng, size, emin=-15, emax=15) -> np.ndarray:
     mant = rng.uniform(1.0, 2.0, size)
     e = rng.integers(emin, emax + 1, size)
     return (sign * np.ldexp(mant, e)).astype(np.float32)
+
+
+def philox4x32_10(ctr, key) -> list[int]:
+    """Philox4x32-10 of one counter (4 words) and key (2 words)."""
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().oracle_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return [int(x) for x in out]
+
+
+def philox_words(blocks: int, seed: int) -> np.ndarray:
+    """Words of blocks 0..blocks-1 (counter (b, 0, 0) split in 32-bit halves, key = seed)."""
+    out = np.zeros(4 * blocks, dtype=np.uint32)
+    lib().oracle_philox_words(_ptr(out), blocks, seed)
+    return out
+
+
+def gaussian(count: int, seed: int, scale: float = 1.0) -> np.ndarray:
+    """FP64 Box-Muller normals of the sketch generator, times scale."""
+    out = np.zeros(count, dtype=np.float64)
+    lib().oracle_gaussian(_ptr(out), count, seed, scale)
+    return out

@@ -18,6 +18,10 @@
  *                    - 14, then hi = fp16(x), lo = fp16(x - hi).
  *   oracle_split_bf16x3 / oracle_split_bf16x2  the BF16 parts of the tf32
  *                    compute type.
+ *   oracle_philox_words / oracle_gaussian  the sketch matrix generator:
+ *                    Philox4x32-10 (Salmon et al., SC'11) words per counter
+ *                    block, Box-Muller normals in FP64 from the same words
+ *                    (the GPU computes them in FP32 and rounds to FP16).
  *
  * Conversions are written from the IEEE 754 binary16/bfloat16 definitions
  * (round to nearest, ties to even; overflow to infinity; gradual underflow).
@@ -299,4 +303,73 @@ int oracle_max_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ---------------------------------------------------------------- */
+/* Sketch generator: Philox4x32-10 + Box-Muller                      */
+/* counter = (b mod 2^32, b / 2^32, 0, 0), key = (seed mod 2^32,     */
+/* seed / 2^32); element 4b + j is normal j of block b.              */
+/* ---------------------------------------------------------------- */
+static void philox_block(uint64_t block, uint64_t seed, uint32_t w[4]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  uint32_t c0 = (uint32_t)block, c1 = (uint32_t)(block >> 32), c2 = 0u, c3 = 0u;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += W0;
+    k1 += W1;
+  }
+  w[0] = c0;
+  w[1] = c1;
+  w[2] = c2;
+  w[3] = c3;
+}
+
+/* Philox4x32-10 with an explicit counter and key (known-answer tests). */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += W0;
+    k1 += W1;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+void oracle_philox_words(uint32_t* out, int64_t blocks, uint64_t seed) {
+  for (int64_t b = 0; b < blocks; ++b) philox_block((uint64_t)b, seed, out + 4 * b);
+}
+
+static double unit_open(uint32_t w) { return ((double)(w >> 9) + 0.5) * (1.0 / 8388608.0); }
+
+void oracle_gaussian(double* out, int64_t count, uint64_t seed, double scale) {
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t b = 0; 4 * b < count; ++b) {
+    uint32_t w[4];
+    philox_block((uint64_t)b, seed, w);
+    double z[4];
+    for (int p = 0; p < 2; ++p) {
+      const double r = sqrt(-2.0 * log(unit_open(w[2 * p])));
+      const double t = two_pi * unit_open(w[2 * p + 1]);
+      z[2 * p] = r * cos(t);
+      z[2 * p + 1] = r * sin(t);
+    }
+    for (int j = 0; j < 4 && 4 * b + j < count; ++j) out[4 * b + j] = z[j] * scale;
+  }
 }

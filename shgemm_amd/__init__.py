@@ -87,6 +87,8 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_group_run.argtypes = [vp, vp, i32]
     L.shgemm_amd_group_plan.argtypes = [vp, ip, ip, ctypes.POINTER(sz)]
     L.shgemm_amd_group_destroy.argtypes = [vp]
+    L.shgemm_amd_gaussian_f16.argtypes = [vp, sz, ctypes.c_uint64, ctypes.c_float, vp]
+    L.shgemm_amd_philox_words.argtypes = [vp, sz, ctypes.c_uint64, vp]
     L.shgemm_amd_last_error.restype = ctypes.c_char_p
     L.shgemm_amd_version.restype = ctypes.c_char_p
     for name in exported_symbols_in_header():
@@ -285,6 +287,23 @@ class Group:
 
     def __exit__(self, *exc):
         self.destroy()
+
+
+# ---- sketch matrices -----------------------------------------------------------
+def _raw_stream(stream):
+    return stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+
+
+def gaussian_f16(out, count: int, seed: int, scale: float = 1.0, stream=None) -> None:
+    """out[i] = FP16(scale * N(0,1)) from Philox4x32-10(seed) + Box-Muller;
+    element i depends only on (seed, i)."""
+    _check(load_library().shgemm_amd_gaussian_f16(_dptr(out), count, seed & (2**64 - 1), float(scale),
+                                                  _raw_stream(stream)))
+
+
+def philox_words(out, blocks: int, seed: int, stream=None) -> None:
+    """out[4b + j] = word j of Philox4x32-10 block b (key = seed)."""
+    _check(load_library().shgemm_amd_philox_words(_dptr(out), blocks, seed & (2**64 - 1), _raw_stream(stream)))
 
 
 def version() -> str:

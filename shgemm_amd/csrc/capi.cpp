@@ -9,6 +9,7 @@
 #include "../../include/shgemm_amd.h"
 #include "errors.hpp"
 #include "plans.hpp"
+#include "registry.hpp"
 
 struct shgemm_amd_handle {
   mtk::shgemm::shgemmHandle_t h;
@@ -245,6 +246,28 @@ int shgemm_amd_group_destroy(shgemm_amd_group* g) {
     if (g == nullptr) return;
     mtk::shgemm::destroy_group(g->g);
     delete g;
+  });
+}
+
+int shgemm_amd_gaussian_f16(uint16_t* out, size_t count, uint64_t seed, float scale, void* stream) {
+  return guarded([&] {
+    if (count == 0) return;
+    if (out == nullptr) throw mtk::shgemm::error("out is null");
+    if (count > ((size_t)1 << 40)) throw mtk::shgemm::error("count too large");
+    const hipError_t e =
+        shgemm_amd::launch_gaussian_f16(out, (int64_t)count, seed, scale, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) throw shgemm_amd::hip_failure(std::string("gaussian_f16: ") + hipGetErrorString(e));
+  });
+}
+
+int shgemm_amd_philox_words(uint32_t* out, size_t blocks, uint64_t seed, void* stream) {
+  return guarded([&] {
+    if (blocks == 0) return;
+    if (out == nullptr) throw mtk::shgemm::error("out is null");
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) throw mtk::shgemm::error("out must be 16-byte aligned");
+    if (blocks > ((size_t)1 << 38)) throw mtk::shgemm::error("blocks too large");
+    const hipError_t e = shgemm_amd::launch_philox_words(out, (int64_t)blocks, seed, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) throw shgemm_amd::hip_failure(std::string("philox_words: ") + hipGetErrorString(e));
   });
 }
 

@@ -122,4 +122,8 @@ hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int ns
                                  hipStream_t s);
 hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int nseg, int64_t blocks, hipStream_t s);
 
+// Sketch generator (sketch_rng.hip).
+hipError_t launch_gaussian_f16(uint16_t* out, int64_t count, uint64_t seed, float scale, hipStream_t s);
+hipError_t launch_philox_words(uint32_t* out, int64_t blocks, uint64_t seed, hipStream_t s);
+
 }  // namespace shgemm_amd

@@ -226,3 +226,47 @@ def test_wide_exponent_generator():
     assert e.min() == -15 and e.max() == 15
     assert (np.abs(a) >= 2.0 ** -15).all() and (np.abs(a) < 2.0 ** 16).all()
     assert 0.45 < (a > 0).mean() < 0.55
+
+
+# Philox4x32-10 known-answer vectors published with the Random123 library
+# (Salmon et al., SC'11): (counter, key) -> output words.
+PHILOX_KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF, 0xFFFFFFFF),
+     (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert tuple(oracle.philox4x32_10(ctr, key)) == want
+
+
+def test_philox_block_counter_and_key_layout():
+    """Block b of seed s is Philox(counter (b lo, b hi, 0, 0), key (s lo, s hi))."""
+    seed = (0x0123456789ABCDEF)
+    w = oracle.philox_words(3, seed)
+    for b in range(3):
+        want = oracle.philox4x32_10((b, 0, 0, 0), (seed & 0xFFFFFFFF, seed >> 32))
+        assert list(w[4 * b:4 * b + 4]) == want
+
+
+def test_gaussian_box_muller_closed_form():
+    """Normal 2p / 2p+1 of a block is the Box-Muller pair of words (2p, 2p+1)."""
+    import math
+
+    seed = 77
+    w = oracle.philox_words(2, seed)
+    z = oracle.gaussian(8, seed, 1.0)
+    u = lambda x: ((int(x) >> 9) + 0.5) / 2.0 ** 23
+    for b in range(2):
+        for p in range(2):
+            r = math.sqrt(-2.0 * math.log(u(w[4 * b + 2 * p])))
+            t = 2.0 * math.pi * u(w[4 * b + 2 * p + 1])
+            assert z[4 * b + 2 * p] == pytest.approx(r * math.cos(t), rel=1e-12, abs=1e-12)
+            assert z[4 * b + 2 * p + 1] == pytest.approx(r * math.sin(t), rel=1e-12, abs=1e-12)
+    big = oracle.gaussian(1 << 20, 5, 0.5)
+    assert abs(big.mean()) < 3e-3 and abs(big.std() - 0.5) < 2e-3

@@ -9,6 +9,7 @@ namespace shgemm_amd {
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
 inline void put_shgemm(KernelTable& t, int variant) {
+  static_assert(CFG::template lds_bytes<MODE>() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
   KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][variant];
   e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OPA, OPB, OPC, MODE, CFG, SPLITK>);
   e.lds = CFG::template lds_bytes<MODE>();
@@ -17,10 +18,12 @@ inline void put_shgemm(KernelTable& t, int variant) {
   e.bk = CFG::BK;
   e.mode = MODE;
   e.threads = CFG::NT;
+  e.wgs_per_cu = (4 * CFG::OCC) / CFG::WAVES > 0 ? (4 * CFG::OCC) / CFG::WAVES : 1;
 }
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG>
 inline void put_grouped(KernelTable& t, int cls) {
+  static_assert(CFG::template lds_bytes<MODE>() + 256 <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
   KernelEntry& e = t.grouped[OPA][OPB][OPC][MODE][cls];
   e.fn = reinterpret_cast<const void*>(&shgemm_grouped_kernel<OPA, OPB, OPC, MODE, CFG>);
   e.lds = CFG::template lds_bytes<MODE>();
@@ -43,6 +46,7 @@ inline void put_shgemm_opc(KernelTable& t) {
   put_shgemm<OPA, OPB, OPC, MODE, typename C::SK, true>(t, V_SK);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, true>(t, V_P1SK);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, true>(t, V_P0SK);
+  if constexpr (C::kHasP2) put_shgemm<OPA, OPB, OPC, MODE, typename C::P2, false>(t, V_P2);
 }
 
 template <int OPA, int OPB, int MODE>

@@ -7,12 +7,16 @@
 
 namespace shgemm_amd {
 
+// LDS of one gfx950 compute unit (the most one workgroup can allocate).
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+
 struct KernelEntry {
   const void* fn = nullptr;
   size_t lds = 0;      // dynamic LDS bytes
   int bm = 0, bn = 0, bk = 0;
   int mode = -1;
   int threads = 256;
+  int wgs_per_cu = 2;  // workgroups one CU holds at the configuration's register budget
 };
 
 enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };
@@ -24,7 +28,8 @@ enum : int {
   V_SK = 2,    // 128 x 128 tile, split-K
   V_P1SK = 3,  // large tile, split-K
   V_P0SK = 4,  // small tile, split-K
-  kNumVariants = 5
+  V_P2 = 5,    // 256 x 256 tile of four 128 x 128 wavefronts (one per SIMD)
+  kNumVariants = 6
 };
 inline bool variant_is_split(int v) { return v == V_SK || v == V_P1SK || v == V_P0SK; }
 inline int variant_level(int v) { return (v == V_P0 || v == V_P0SK) ? LEVEL_P0 : LEVEL_P1; }

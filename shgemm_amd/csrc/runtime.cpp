@@ -76,8 +76,9 @@ double plan_cost(const shgemm_amd::KernelEntry& e, int variant, int mode, int64_
   const int level = shgemm_amd::variant_level(variant);
   // workgroups resident per CU and the effective MFMA rate one CU sustains on
   // this tile (TFLOP/s effective per CU), from the tile sweep
-  const double per_cu_tflops = (mode == shgemm_amd::MODE_BF16 ? 0.8 : 1.8) * (level == 1 ? 1.0 : 0.55);
-  const int wg_per_cu = (e.threads >= 512) ? 2 : (level == 1 ? 2 : 4);
+  const double per_cu_tflops = (mode == shgemm_amd::MODE_BF16 ? 0.8 : 1.8) * (level == 1 ? 1.0 : 0.55) *
+                               (variant == shgemm_amd::V_P2 ? 1.03 : 1.0);
+  const int wg_per_cu = level == 1 ? e.wgs_per_cu : 4;
   const int64_t wgs = tiles * slices;
   const int64_t waves = (wgs + (int64_t)ncu * wg_per_cu - 1) / ((int64_t)ncu * wg_per_cu);
   const double wg_flops = 2.0 * e.bm * e.bn * ((double)k / slices);
@@ -98,7 +99,7 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
-// A plan variant 0..5 with a slice count -> a launchable plan.  Split
+// A plan variant 0..6 with a slice count -> a launchable plan.  Split
 // variants get 2..max_slices slices; when k or the workspace allows no split
 // they run as the unsplit variant of the same tile.
 Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants],
@@ -120,6 +121,18 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
   }
   pl.level = pl.slices > 1 ? detail::P0 : shgemm_amd::variant_level(pl.variant);
   return pl;
+}
+
+// MTK_SHGEMM_TUNE_P1 (tile sweep): an unsplit large-tile op_n/op_n/op_n plan
+// runs the selected candidate configuration instead.
+bool apply_tile_sweep(const detail::handle_state* st, Plan& pl, int mode, int oa, int ob, int oc) {
+  if (st->tune_p1 < 0 || pl.slices != 1 || pl.variant != shgemm_amd::V_P1 || pl.entry != nullptr) return false;
+  if (st->table.tune[st->tune_p1].mode != mode || oa != shgemm_amd::OP_N || ob != shgemm_amd::OP_N ||
+      oc != shgemm_amd::OP_N)
+    return false;
+  pl.entry = &st->table.tune[st->tune_p1];
+  pl.note = " tile-sweep candidate";
+  return true;
 }
 
 Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
@@ -145,6 +158,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
           shgemm_amd::find_plan(st->plans, shgemm_amd::PLAN_SINGLE, mode, oa, ob, oc, m, n, k)) {
     Plan pl = resolve_plan(st, tab, r->choice, r->slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
     pl.source = plan_table;
+    apply_tile_sweep(st, pl, mode, oa, ob, oc);
     return pl;
   }
   // 4. cost model over every variant and split count
@@ -170,10 +184,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   best.source = plan_cost_model;
   // tile-sweep candidate or deep-prefetch kernel for the large-tile fp16 path
   if (best.slices == 1 && best.variant == shgemm_amd::V_P1) {
-    if (st->tune_p1 >= 0 && st->table.tune[st->tune_p1].mode == mode && oa == shgemm_amd::OP_N &&
-        ob == shgemm_amd::OP_N && oc == shgemm_amd::OP_N) {
-      best.entry = &st->table.tune[st->tune_p1];
-      best.note = " tile-sweep candidate";
+    if (apply_tile_sweep(st, best, mode, oa, ob, oc)) {
     } else if (st->use_bd) {
       best.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
       if (best.entry != nullptr) {

@@ -397,17 +397,25 @@ struct TileCfg {
 // split/store phase of one with the MFMA phase of another).  The
 // two-accumulator modes use 4 wavefronts of 64 x 64.  Chosen by the tile
 // sweep at m = n = k = 8192 and 16384.
+// P2 (one-accumulator modes): a 256 x 256 tile of four 128 x 128 wavefronts,
+// one per SIMD, accumulators in AGPRs -- half the LDS operand traffic per
+// MFMA of the 64 x 64 wavefront tiles; the tile sweep measured it fastest at
+// m = n = k = 8192 and 16384.  The two-accumulator mode has no room for it.
 template <int MODE>
 struct ModeCfg {
   using P1 = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+  using P2 = TileCfg<256, 256, 32, 128, 128, 1, 1>;
+  static constexpr bool kHasP2 = (MODE == MODE_BF16);
 };
 template <>
 struct ModeCfg<MODE_FP16S> {
-  using P1 = TileCfg<128, 256, 32, 64, 64, 1, 2>;  // 8 wavefronts, 4 per SIMD
+  using P1 = TileCfg<128, 256, 32, 64, 64, 1, 4>;  // 8 wavefronts, 2 workgroups (4 wavefronts) per SIMD
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+  using P2 = TileCfg<256, 256, 32, 128, 128, 1, 1>;
+  static constexpr bool kHasP2 = true;
 };
 
 // One BK-deep k-tile of MFMA work for one wavefront: fragments from the A
@@ -615,7 +623,26 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       sstore(S0{}, 0);
     }
     __syncthreads();
-    for (int kt = 0; kt < ktiles; ++kt) {
+    int kt = 0;
+    // Interior tiles while the next k-tile is full: only the unchecked loads,
+    // with no branch between the A and B loads, so the compiler neither
+    // merges the checked paths' register state into the loop (which made it
+    // wait for the A loads before issuing the B loads) nor splits the loads
+    // across blocks.
+    const int kfull = (kend - kbeg) / BK;
+    if (a_full && b_full) {
+      for (; kt + 1 < kfull; ++kt) {
+        const int cur = kt & 1;
+        const int k1 = kbeg + (kt + 1) * BK;
+        ta[0].load_fast(A, p.lda, m0, k1, tid);
+        tb[0].load_fast(B, p.ldb, n0, k1, tid);
+        __builtin_amdgcn_sched_barrier(0);  // loads go out before the MFMA work
+        compute(lds + cur * STAGE);
+        sstore(S0{}, cur ^ 1);
+        __syncthreads();
+      }
+    }
+    for (; kt < ktiles; ++kt) {
       const int cur = kt & 1;
       if (kt + 1 < ktiles) gload(S0{}, kt + 1);
       compute(lds + cur * STAGE);

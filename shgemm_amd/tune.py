@@ -7,8 +7,8 @@ measurements on the device at hand.
 
 Two kinds of rows (format in csrc/plans.hpp):
   single   one shgemm call at shape point (2^lm, 2^ln, 2^lk): every plan
-           variant (P0, P1, P1 three-stage, and the split-K variants at 2..256
-           slices) is timed alone; the fastest is kept.
+           variant (P0, P1, P2, P1 three-stage, and the split-K variants at
+           2..256 slices) is timed alone; the fastest is kept.
   grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
            2 large) x k-slices {1, 2, 4} are timed on a group of identical
            copies large enough to fill the GPU several times over, so the
@@ -111,7 +111,7 @@ def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
         B = torch.randn(k * n, device="cuda").half()
         C = torch.empty(m * n, device="cuda")
         seen, best = set(), None
-        candidates = [(0, 1), (1, 1), (5, 1)] + [(v, s) for v in (2, 3, 4) for s in (2, 4, 8, 16, 32, 64, 128, 256)]
+        candidates = [(0, 1), (1, 1), (5, 1), (6, 1)] + [(v, s) for v in (2, 3, 4) for s in (2, 4, 8, 16, 32, 64, 128, 256)]
         for v, s in candidates:
             sg.force_plan(h, v, s)
             eff = sg.query_plan(h, oa, ob, m, n, k, ct, oc)[:2]
@@ -142,8 +142,11 @@ def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
     for m in gm:
         for n in gn:
             for k in gk:
-                A = torch.rand(m * k, device="cuda") * 2 - 1
-                B = torch.randn(k * n, device="cuda").half()
+                # distinct operands per copy (up to ~4 GB of A), so copies running
+                # side by side do not share cache lines the real batch would not
+                n_a = max(1, min(GROUP_MAX_COPIES, int(4e9 // (4 * m * k))))
+                As = [torch.rand(m * k, device="cuda") * 2 - 1 for _ in range(n_a)]
+                Bs = [torch.randn(k * n, device="cuda").half() for _ in range(n_a)]
                 best = None
                 for cls in (0, 1, 2):
                     for s in GROUP_SLICES:
@@ -151,7 +154,7 @@ def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
                         tiles = _tiles(cls, mode, m, n)
                         copies = max(1, min(GROUP_MAX_COPIES, GROUP_UNITS // max(1, tiles * s)))
                         C = torch.empty(copies * m * n, device="cuda")
-                        probs = [(m, n, k, A, m, B, k, C[i * m * n:(i + 1) * m * n], m, 1.0, 0.0)
+                        probs = [(m, n, k, As[i % n_a], m, Bs[i % n_a], k, C[i * m * n:(i + 1) * m * n], m, 1.0, 0.0)
                                  for i in range(copies)]
                         with sg.Group(h, sg.op_n, sg.op_n, probs, ct, tile_class=[cls] * copies,
                                       slices=[s] * copies) as grp:
@@ -167,7 +170,7 @@ def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
                 rows.append(f"grouped\t{mode}\tN\tN\tN\t{lm}\t{ln}\t{lk}\t{cls}\t{s}\t{t * 1e3:.2f}")
                 log(f"grouped {mode} m={m} n={n} k={k}: class {cls} x{s} {t * 1e3:.2f} us/problem "
                     f"({2.0 * m * n * k / t / 1e9:.1f} TFLOP/s)")
-                del A, B
+                del As, Bs
                 torch.cuda.empty_cache()
     return rows
 
@@ -207,7 +210,7 @@ def main(argv=None) -> int:
         f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}, {props.multi_processor_count} CUs)",
         f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
         f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
-        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P1 three-stage);"
+        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P1 three-stage);"
         " time_us per call",
         "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large); time_us per problem in a full group",
     ]

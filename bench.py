@@ -31,9 +31,10 @@ PEAK_HBM = 8.0             # TB/s
 
 def parse():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["square", "grouped", "serve"], default="square",
+    ap.add_argument("--workload", choices=["square", "grouped", "serve", "configs"], default="square",
                     help="square: m=n=k=--size (the headline line); grouped: 512 uneven problems in one group; "
-                         "serve: sketch server under Poisson arrivals")
+                         "serve: sketch server under Poisson arrivals; configs: the 1024^3, sketch and "
+                         "tall-skinny configurations, one JSON line each")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -312,6 +313,149 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def _time_ms(fn, iters, stream):
+    """Median per-call milliseconds over 3 batches of `iters` calls (HIP events on `stream`)."""
+    import statistics
+
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    out = []
+    for _ in range(3):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(iters):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        out.append(s.elapsed_time(e) / iters)
+    return statistics.median(out)
+
+
+def bench_configs(args):
+    """The remaining configurations, each timed on its own inputs with the
+    plan the library picks, its FP64 error on sampled rows beside the
+    library SGEMM's, and its roofline fraction:
+      cfg1    1024^3 op_n/op_n, A U[-1,1], B FP16 U[-1,1]          (MFMA-bound, launch-latency dominated)
+      sketch  A 16384^2 = sign*U[1,2)*2^e, e in [-15,15]; Omega 16384x256 FP16 N(0,1)/16
+              shgemm nn, shgemm tn (A stored transposed), hsgemm (Y^T = Omega^T A^T); fp16 and tf32
+      cfg3    m = n = 128, k = 2^20, A U[-1,1], B FP16 N(0,1)      (HBM-bound: 4mk + 2kn + 4mn bytes)"""
+    import numpy as np
+    import torch
+
+    import shgemm_amd as sg
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    h = sg.create()
+    sg.set_cuda_stream(h, stream)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    rng = np.random.default_rng(args.seed)
+
+    def emit(name, m, n, k, ms, err, err_s, extra):
+        flops = 2.0 * m * n * k
+        byts = 4.0 * m * k + 2.0 * k * n + 4.0 * m * n
+        t_mfma = flops / (PEAK_FP16_DENSE / 2.0 * 1e12) * (2.5 if extra.get("compute_type") == "tf32" else 1.0)
+        t_hbm = byts / (PEAK_HBM * 1e12)
+        line = {"config": name, "m": m, "n": n, "k": k, "ms": round(ms, 5),
+                "tflops_eff": round(flops / (ms * 1e-3) / 1e12, 2),
+                "gbps_alg": round(byts / (ms * 1e-3) / 1e9, 1),
+                "bound": "mfma" if t_mfma > t_hbm else "hbm",
+                "roofline_frac": round(max(t_mfma, t_hbm) / (ms * 1e-3), 4),
+                "err": err, "sgemm_err": err_s, "err_ratio": err / err_s}
+        line.update(extra)
+        print(json.dumps(line), flush=True)
+
+    def rows_err(A_rows, B64, C_rows):
+        ref = A_rows.astype(np.float64) @ B64
+        return float(np.linalg.norm(C_rows.astype(np.float64) - ref) / np.linalg.norm(ref))
+
+    # ---- cfg1 ---------------------------------------------------------------
+    m = n = k = 1024
+    A = torch.rand(m * k, device=dev, generator=g) * 2 - 1
+    B = (torch.rand(k * n, device=dev, generator=g) * 2 - 1).half()
+    C = torch.empty(m * n, device=dev)
+    Cs = torch.empty(m * n, device=dev)
+    ms = _time_ms(lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16), 50, stream)
+    sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B.float(), k, 0.0, Cs, m)
+    torch.cuda.synchronize()
+    An = A.view(k, m).t().cpu().numpy()
+    Bn = B.float().view(n, k).t().cpu().numpy().astype(np.float64)
+    plan = sg.query_plan(h, 0, 0, m, n, k, sg.fp16)
+    emit("cfg1 1024^3 nn fp16", m, n, k, ms, rows_err(An, Bn, C.view(n, m).t().cpu().numpy()),
+         rows_err(An, Bn, Cs.view(n, m).t().cpu().numpy()),
+         {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[plan[0]], plan[1]]})
+    del A, B, C, Cs
+
+    # ---- sketch (wide exponent) -----------------------------------------------
+    m, k, l = 16384, 16384, 256
+    e = torch.randint(-15, 16, (m * k,), device=dev, generator=g).float()
+    sgn = torch.where(torch.rand(m * k, device=dev, generator=g) < 0.5, -1.0, 1.0)
+    A = sgn * (1.0 + torch.rand(m * k, device=dev, generator=g)) * torch.exp2(e)
+    del e, sgn
+    At = A.view(k, m).t().contiguous().view(-1)   # A stored transposed (k x m col-major = m x k row-major)
+    Om = (torch.randn(k * l, device=dev, generator=g) / 16.0).half()
+    Om32 = Om.float()
+    rows = np.sort(rng.choice(m, size=256, replace=False))
+    ridx = torch.from_numpy(rows).to(dev)
+    A_rows = A.view(k, m).t()[ridx].cpu().numpy()
+    Om64 = Om32.view(l, k).t().cpu().numpy().astype(np.float64)
+    Ys = torch.empty(m * l, device=dev)
+    sg.sgemm_fp32(h, 0, 0, m, l, k, 1.0, A, m, Om32, k, 0.0, Ys, m)
+    torch.cuda.synchronize()
+    err_s = rows_err(A_rows, Om64, Ys.view(l, m).t()[ridx].cpu().numpy())
+    Y = torch.empty(m * l, device=dev)
+    for ct, ctn in ((sg.fp16, "fp16"), (sg.tf32, "tf32")):
+        for variant in ("shgemm nn", "shgemm tn", "hsgemm"):
+            if variant == "shgemm nn":
+                call = lambda: sg.shgemm(h, 0, 0, m, l, k, 1.0, A, m, Om, k, 0.0, Y, m, ct)
+                q = sg.query_plan(h, 0, 0, m, l, k, ct)
+            elif variant == "shgemm tn":
+                call = lambda: sg.shgemm(h, 1, 0, m, l, k, 1.0, At, k, Om, k, 0.0, Y, m, ct)
+                q = sg.query_plan(h, 1, 0, m, l, k, ct)
+            else:  # Y^T (l x m) = Omega^T A^T: hsgemm(op_t, op_t) with A16 = Omega, B32 = A
+                call = lambda: sg.hsgemm(h, 1, 1, l, m, k, 1.0, Om, k, A, m, 0.0, Y, l, ct)
+                q = sg.query_plan(h, 0, 0, m, l, k, ct, 1)
+            ms = _time_ms(call, 10, stream)
+            torch.cuda.synchronize()
+            Yh = Y.view(m, l)[ridx] if variant == "hsgemm" else Y.view(l, m).t()[ridx]
+            emit(f"sketch {variant} {ctn}", m, l, k, ms, rows_err(A_rows, Om64, Yh.cpu().numpy()), err_s,
+                 {"compute_type": ctn, "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "prescaled": ctn == "fp16"})
+    del A, At, Om, Om32, Y, Ys
+    torch.cuda.empty_cache()
+
+    # ---- cfg3 tall-skinny ------------------------------------------------------
+    m = n = 128
+    k = 1 << 20
+    A = torch.rand(m * k, device=dev, generator=g) * 2 - 1
+    B = torch.randn(k * n, device=dev, generator=g).half()
+    C = torch.empty(m * n, device=dev)
+    Cs = torch.empty(m * n, device=dev)
+    for scaling in (True, False):
+        sg.set_exponent_scaling(h, scaling)
+        ms = _time_ms(lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16), 20, stream)
+        runs = []
+        for _ in range(10):
+            sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16)
+            runs.append(C.clone())
+        torch.cuda.synchronize()
+        identical = all(torch.equal(runs[0].view(torch.int32), r.view(torch.int32)) for r in runs[1:])
+        if scaling:
+            sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B.float(), k, 0.0, Cs, m)
+            torch.cuda.synchronize()
+            An = A.view(k, m).t().cpu().numpy()
+            Bn = B.float().view(n, k).t().cpu().numpy().astype(np.float64)
+            err_s = rows_err(An, Bn, Cs.view(n, m).t().cpu().numpy())
+        q = sg.query_plan(h, 0, 0, m, n, k, sg.fp16)
+        emit(f"cfg3 128x128x2^20 fp16 {'prescaled' if scaling else 'classic split'}", m, n, k, ms,
+             rows_err(An, Bn, C.view(n, m).t().cpu().numpy()), err_s,
+             {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "bit_identical_10_runs": identical})
+    sg.set_exponent_scaling(h, True)
+    sg.destroy(h)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -323,6 +467,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "configs":
+        bench_configs(args)
         return
 
     import numpy as np

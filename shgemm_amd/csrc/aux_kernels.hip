@@ -1,5 +1,7 @@
-// Auxiliary kernels: deterministic split-K reduction, beta-only update and the
-// per-row exponent pass of the prescaled fp16 mode.
+// Auxiliary kernels: deterministic split-K reduction, beta-only update, the
+// per-row exponent pass of the prescaled fp16 mode (single call: k-block
+// partial maxima + a reduction; grouped: one pass with atomic maxima) and
+// their grouped forms.
 #include <algorithm>
 #include "registry.hpp"
 
@@ -59,10 +61,14 @@ __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b 
 // A workgroup covers `rq` row-quads (4 consecutive rows each) and 256/rq
 // k-lanes, so narrow matrices (small m) still keep every thread loading.
 // With `vec` a row-quad at one k is one 16-byte load (rows 4-aligned,
-// lda % 4 == 0, 16-byte base).  grid.x covers row-quads in groups of rq,
-// grid.y covers k in chunks of kchunk; the max is order-independent.
+// lda % 4 == 0, 16-byte base); eight such loads are issued before they are
+// reduced, so a thread keeps 128 bytes in flight.  grid.x covers row-quads in
+// groups of rq, grid.y covers k in chunks of kchunk.  ATOMIC: the maxima go
+// to dst[r] with atomicMax (order-independent); otherwise block y writes its
+// partial maxima to dst[r] (dst already offset to its slice of the partials).
+template <bool ATOMIC>
 __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                     int kchunk, int rq, int vec, uint32_t* __restrict__ row_max,
+                                                     int kchunk, int rq, int vec, uint32_t* __restrict__ dst,
                                                      int bx, int by) {
   const int q = threadIdx.x % rq, kl = threadIdx.x / rq, klanes = 256 / rq;
   const int r4 = (bx * rq + q) * 4;
@@ -70,13 +76,21 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
   if (r4 < m && kl < klanes) {
     const int k0 = by * kchunk, k1 = min(k, k0 + kchunk);
     if (vec && r4 + 4 <= m) {
-#pragma unroll 4
-      for (int kk = k0 + kl; kk < k1; kk += klanes) {
-        const float4 v = *reinterpret_cast<const float4*>(a + r4 + (int64_t)kk * lda);
-        mx[0] = umax(mx[0], abs_bits(v.x));
-        mx[1] = umax(mx[1], abs_bits(v.y));
-        mx[2] = umax(mx[2], abs_bits(v.z));
-        mx[3] = umax(mx[3], abs_bits(v.w));
+      constexpr int U = 8;
+      for (int kk = k0 + kl; kk < k1; kk += U * klanes) {
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kx = kk + u * klanes;
+          v[u] = kx < k1 ? *reinterpret_cast<const float4*>(a + r4 + (int64_t)kx * lda) : make_float4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          mx[0] = umax(mx[0], abs_bits(v[u].x));
+          mx[1] = umax(mx[1], abs_bits(v[u].y));
+          mx[2] = umax(mx[2], abs_bits(v[u].z));
+          mx[3] = umax(mx[3], abs_bits(v[u].w));
+        }
       }
     } else {
       for (int kk = k0 + kl; kk < k1; kk += klanes)
@@ -84,7 +98,7 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
           if (r4 + e < m) mx[e] = umax(mx[e], abs_bits(a[r4 + e + (int64_t)kk * lda]));
     }
   }
-  // combine the k-lanes of each row-quad in LDS, then one atomic per row
+  // combine the k-lanes of each row-quad in LDS, then one store (or atomic) per row
   __shared__ uint32_t red[4][256];
   for (int e = 0; e < 4; ++e) red[e][threadIdx.x] = mx[e];
   __syncthreads();
@@ -92,22 +106,27 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
     for (int e = 0; e < 4; ++e) {
       uint32_t v = 0;
       for (int j = 0; j < klanes; ++j) v = umax(v, red[e][q + j * rq]);
-      if (r4 + e < m) atomicMax(&row_max[r4 + e], v);
+      if (r4 + e < m) {
+        if constexpr (ATOMIC) atomicMax(&dst[r4 + e], v);
+        else dst[r4 + e] = v;
+      }
     }
   }
 }
 
 __global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
                                                               int kchunk, int rq, int vec,
-                                                              uint32_t* __restrict__ row_max) {
-  rowmax_rcontig_block(a, lda, m, k, kchunk, rq, vec, row_max, blockIdx.x, blockIdx.y);
+                                                              uint32_t* __restrict__ partial) {
+  rowmax_rcontig_block<false>(a, lda, m, k, kchunk, rq, vec, partial + (int64_t)blockIdx.y * m, blockIdx.x,
+                              blockIdx.y);
 }
 
 // A stored with k contiguous (op_t): element (r, k) at a[r*lda + k].
 // One workgroup per (row, k chunk): 16-byte loads when `vec`, a block
 // reduction, then one atomic.
+template <bool ATOMIC>
 __device__ __forceinline__ void rowmax_kcontig_block(const float* __restrict__ a, int64_t lda, int k, int kchunk,
-                                                     int vec, uint32_t* __restrict__ row_max, int r, int by) {
+                                                     int vec, uint32_t* __restrict__ dst, int r, int by) {
   const int k0 = by * kchunk, k1 = min(k, k0 + kchunk);
   const float* row = a + (int64_t)r * lda;
   uint32_t mx = 0;
@@ -130,28 +149,59 @@ __device__ __forceinline__ void rowmax_kcontig_block(const float* __restrict__ a
     if (threadIdx.x < s) red[threadIdx.x] = umax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicMax(&row_max[r], red[0]);
+  if (threadIdx.x == 0) {
+    if constexpr (ATOMIC) atomicMax(&dst[r], red[0]);
+    else dst[r] = red[0];
+  }
 }
 
 __global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
-                                                              int kchunk, int vec, uint32_t* __restrict__ row_max) {
-  (void)m;
-  rowmax_kcontig_block(a, lda, k, kchunk, vec, row_max, blockIdx.x, blockIdx.y);
+                                                              int kchunk, int vec, uint32_t* __restrict__ partial) {
+  rowmax_kcontig_block<false>(a, lda, k, kchunk, vec, partial + (int64_t)blockIdx.y * m, blockIdx.x, blockIdx.y);
 }
 
-__global__ void __launch_bounds__(256) rowexp_kernel(const uint32_t* __restrict__ row_max, int64_t m,
-                                                      int* __restrict__ row_exp) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= m) return;
-  const uint32_t b = row_max[r];
+// row_exp = ilogb(max |A[r,:]|) - 14 from the |a| bit pattern of the maximum
+// (0 for an all-zero or non-finite row), clamped so 2^-e stays a finite FP32.
+__device__ __forceinline__ int exponent_of_max(uint32_t b) {
   int e = 0;
   if (b != 0 && b < 0x7f800000u) {  // finite, non-zero
     const int biased = (int)(b >> 23);
     const int lg = biased != 0 ? biased - 127 : (31 - __builtin_clz(b)) - 149;  // ilogb(|a|max)
     e = lg - 14;
-    e = e < -126 ? -126 : e;  // keep 2^-e a finite FP32 scale
+    e = e < -126 ? -126 : e;
   }
-  row_exp[r] = e;
+  return e;
+}
+
+__global__ void __launch_bounds__(256) rowexp_kernel(const uint32_t* __restrict__ row_max, int64_t m,
+                                                      int* __restrict__ row_exp) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < m) row_exp[r] = exponent_of_max(row_max[r]);
+}
+
+// Reduction of kb partial maxima per row (in[b*m + r]) in groups of 256: a
+// block covers 32 rows with 8 lanes over one group of partials and combines
+// them in LDS.  With one group (gridDim.y == 1) it writes the row exponents;
+// otherwise it writes one maximum per group to out[blockIdx.y*m + r] for the
+// next level.
+__global__ void __launch_bounds__(256) rowexp_reduce_kernel(const uint32_t* __restrict__ in, int kb, int m,
+                                                             uint32_t* __restrict__ out, int* __restrict__ row_exp) {
+  const int rl = threadIdx.x % 32, kl = threadIdx.x / 32;
+  const int r = blockIdx.x * 32 + rl;
+  const int b0 = blockIdx.y * 256, b1 = min(kb, b0 + 256);
+  uint32_t mx = 0;
+  if (r < m) {
+#pragma unroll 8
+    for (int b = b0 + kl; b < b1; b += 8) mx = umax(mx, in[(int64_t)b * m + r]);
+  }
+  __shared__ uint32_t red[8][32];
+  red[kl][rl] = mx;
+  __syncthreads();
+  if (kl == 0 && r < m) {
+    for (int j = 1; j < 8; ++j) mx = umax(mx, red[j][rl]);
+    if (gridDim.y == 1) row_exp[r] = exponent_of_max(mx);
+    else out[(int64_t)blockIdx.y * m + r] = mx;
+  }
 }
 
 
@@ -189,23 +239,69 @@ RowmaxGeometry rowmax_geometry(int op_a, int m, int k) {
   return g;
 }
 
-hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
-                                int* row_exp, hipStream_t s) {
-  hipError_t err = launch_fill_u32(row_max, m, 0u, s);
-  if (err != hipSuccess) return err;
-  const bool aligned16 = (reinterpret_cast<uintptr_t>(a) % 16) == 0 && (lda % 4) == 0;
-  if (k > 0) {
-    const RowmaxGeometry g = rowmax_geometry(op_a, m, k);
-    if (op_a == OP_N)
-      hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
-                         g.kchunk, g.rq, aligned16 ? 1 : 0, row_max);
-    else
-      hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
-                         g.kchunk, aligned16 ? 1 : 0, row_max);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
+// Single-call geometry: about 2048 workgroups (enough to spread over every
+// CU), at most 4096 k-blocks (each writes one partial maximum per row), and
+// the partials plus one reduction level inside `partial_cap` words.  A
+// k-block count of 1 writes the maxima directly.
+static RowmaxGeometry rowmax_partial_geometry(int op_a, int m, int k, int64_t partial_cap) {
+  RowmaxGeometry g{};
+  int64_t rb;
+  int kb_min_chunk;
+  if (op_a == OP_N) {
+    const int quads = (m + 3) / 4;
+    g.rq = 1;
+    while (g.rq < 256 && g.rq < quads) g.rq *= 2;
+    g.bx = (quads + g.rq - 1) / g.rq;
+    rb = g.bx;
+    kb_min_chunk = 256 / g.rq;  // one load per k-lane at least
+  } else {
+    g.rq = 0;
+    g.bx = m;
+    rb = m;
+    kb_min_chunk = 4096;  // 256 threads x 16 values
   }
-  return launch_row_exp(row_max, m, row_exp, s);
+  int64_t kb = std::max<int64_t>(1, std::min<int64_t>(4096, (2048 + rb - 1) / rb));
+  kb = std::min<int64_t>(kb, std::max<int64_t>(1, (partial_cap / std::max(1, m)) * 256 / 257));
+  kb = std::min<int64_t>(kb, std::max<int64_t>(1, (k + kb_min_chunk - 1) / kb_min_chunk));
+  g.kchunk = (int)((k + kb - 1) / kb);
+  g.by = (k + g.kchunk - 1) / g.kchunk;
+  return g;
+}
+
+hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
+                                int* row_exp, uint32_t* partials, size_t partial_bytes, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  if (k <= 0) {
+    hipError_t err = launch_fill_u32(row_max, m, 0u, s);
+    return err != hipSuccess ? err : launch_row_exp(row_max, m, row_exp, s);
+  }
+  const bool aligned16 = (reinterpret_cast<uintptr_t>(a) % 16) == 0 && (lda % 4) == 0;
+  const RowmaxGeometry g = rowmax_partial_geometry(op_a, m, k, (int64_t)(partial_bytes / sizeof(uint32_t)));
+  uint32_t* dst = g.by == 1 ? row_max : partials;
+  if (op_a == OP_N)
+    hipLaunchKernelGGL(rowmax_rcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
+                       g.kchunk, g.rq, aligned16 ? 1 : 0, dst);
+  else
+    hipLaunchKernelGGL(rowmax_kcontig_kernel, dim3((unsigned)g.bx, (unsigned)g.by), dim3(256), 0, s, a, lda, m, k,
+                       g.kchunk, aligned16 ? 1 : 0, dst);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  if (g.by == 1) return launch_row_exp(row_max, m, row_exp, s);
+  // levels of 256-way reduction until one group remains; level outputs follow
+  // the partials in the same scratch
+  const unsigned rblocks = (unsigned)((m + 31) / 32);
+  uint32_t* in = partials;
+  int kb = g.by;
+  while (true) {
+    const int groups = (kb + 255) / 256;
+    uint32_t* out = in + (int64_t)kb * m;
+    hipLaunchKernelGGL(rowexp_reduce_kernel, dim3(rblocks, (unsigned)groups), dim3(256), 0, s, in, kb, m, out,
+                       row_exp);
+    err = hipGetLastError();
+    if (err != hipSuccess || groups == 1) return err;
+    in = out;
+    kb = groups;
+  }
 }
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
@@ -242,8 +338,9 @@ __global__ void __launch_bounds__(256) grouped_rowmax_kernel(const RowmaxSeg* __
   const RowmaxSeg sg = segs[i];
   const int local = b - sg.block_begin;
   const int bx = local % sg.bx, by = local / sg.bx;
-  if constexpr (OPA == OP_N) rowmax_rcontig_block(sg.a, sg.lda, sg.m, sg.k, sg.kchunk, sg.rq, sg.vec, sg.row_max, bx, by);
-  else rowmax_kcontig_block(sg.a, sg.lda, sg.k, sg.kchunk, sg.vec, sg.row_max, bx, by);
+  if constexpr (OPA == OP_N)
+    rowmax_rcontig_block<true>(sg.a, sg.lda, sg.m, sg.k, sg.kchunk, sg.rq, sg.vec, sg.row_max, bx, by);
+  else rowmax_kcontig_block<true>(sg.a, sg.lda, sg.k, sg.kchunk, sg.vec, sg.row_max, bx, by);
 }
 
 __global__ void __launch_bounds__(256) grouped_reduce_kernel(const ReduceSeg* __restrict__ segs,

@@ -119,8 +119,10 @@ RowmaxGeometry rowmax_geometry(int op_a, int m, int k);
 hipError_t launch_splitk_reduce(const float* ws, int slices, int64_t elems, int inner, float alpha, float beta,
                                 float* c, int64_t ldc, hipStream_t s);
 hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_t ldc, hipStream_t s);
+// Row exponents of A for the prescaled mode; `partials` (partial_bytes) holds
+// the k-block maxima, so calls on one stream must not share it concurrently.
 hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
-                                int* row_exp, hipStream_t s);
+                                int* row_exp, uint32_t* partials, size_t partial_bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s);
 hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,

@@ -402,7 +402,10 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
       check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
       st->row_cap = m;
     }
-    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp, stream),
+    // the k-block partial maxima live in the split-K workspace: this call's
+    // GEMM uses it only after the exponent pass, in stream order
+    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp,
+                                               reinterpret_cast<uint32_t*>(st->ws), st->ws_bytes, stream),
               "row exponent pass");
     p.row_exp = st->row_exp;
   }

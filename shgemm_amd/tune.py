@@ -37,14 +37,14 @@ MODES = {  # plan-table mode -> (compute type, exponent scaling)
 }
 OPS = {"N": sg.op_n, "T": sg.op_t}
 
-SINGLE_M = [128, 512, 2048, 8192, 16384]
-SINGLE_N = [64, 256, 1024, 4096, 16384]
+SINGLE_M = [128, 512, 1024, 2048, 8192, 16384]
+SINGLE_N = [64, 256, 1024, 4096, 8192, 16384]
 SINGLE_K = [256, 1024, 4096, 16384, 65536, 262144, 1048576]
 QUICK_M = [128, 2048, 16384]
 QUICK_N = [64, 1024, 16384]
 QUICK_K = [1024, 16384, 1048576]
-MAX_BYTES = 2.5e9       # operands + C of one tuning shape
-MAX_FLOPS = 2.0 ** 41   # 2mnk of one tuning shape
+MAX_BYTES = 3.0e9       # operands + C of one tuning shape
+MAX_FLOPS = 2.0 ** 43   # 2mnk of one tuning shape (m = n = k = 16384 included)
 
 GROUP_M = [256, 1024, 4096, 8192]
 GROUP_N = [64, 128, 256, 512]
@@ -182,9 +182,50 @@ def _tiles(cls: int, mode: str, m: int, n: int) -> int:
     return -(-m // bm) * -(-n // bn)
 
 
+def sweep_tiles(candidates, sizes, log) -> None:
+    """Times the compiled tile candidates of the op_n/op_n fp16 kernel
+    (csrc/inst_tune.hip; MTK_SHGEMM_TUNE_P1=<i> is read when a handle is
+    created) at m = n = k = each size; -1 is the library's own plan."""
+    import os
+
+    torch = _torch()
+    for N in sizes:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1)
+        A = torch.rand(N * N, device="cuda", generator=g) * 2 - 1
+        B = (torch.rand(N * N, device="cuda", generator=g) * 2 - 1).half()
+        C = torch.empty(N * N, device="cuda")
+        ref = None
+        for v in candidates:
+            saved = {k: os.environ.get(k) for k in ("MTK_SHGEMM_TUNE_P1", "MTK_SHGEMM_EXP_SCALING")}
+            if v >= 0:
+                os.environ["MTK_SHGEMM_TUNE_P1"] = str(v)
+            else:
+                os.environ.pop("MTK_SHGEMM_TUNE_P1", None)
+            os.environ["MTK_SHGEMM_EXP_SCALING"] = "0" if v == 0 else "1"  # candidate 0 is the two-accumulator mode
+            h = sg.create()
+            for k_, val in saved.items():
+                if val is None:
+                    os.environ.pop(k_, None)
+                else:
+                    os.environ[k_] = val
+            t = time_fn(lambda: sg.shgemm(h, sg.op_n, sg.op_n, N, N, N, 1.0, A, N, B, N, 0.0, C, N, sg.fp16))
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = C.clone()
+                d = 0.0
+            else:
+                d = ((C - ref).norm() / ref.norm()).item()
+            log(f"N={N} candidate={v:3d} {t:.3f} ms {2.0 * N ** 3 / t / 1e9:7.1f} TFLOP/s eff  reldiff_vs_first={d:.2e}")
+            sg.destroy(h)
+        del A, B, C
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="plan table to write")
+    ap.add_argument("--sweep-tiles", default="", help="comma list of tile candidates to time instead of tuning")
+    ap.add_argument("--sizes", default="4096,8192,16384", help="sizes for --sweep-tiles")
     ap.add_argument("--quick", action="store_true", help="reduced shape grid")
     ap.add_argument("--single", default="fp16s:NNN,fp16s:TNN,fp16s:TTT,fp16:NNN,bf16:NNN",
                     help="mode:ops list for single-call rows ('' for none)")
@@ -193,6 +234,11 @@ def main(argv=None) -> int:
     torch = _torch()
     t0 = time.time()
     log = lambda msg: print(f"[tune {time.time() - t0:6.1f}s] {msg}", flush=True)
+    if args.sweep_tiles:
+        sweep_tiles([int(x) for x in args.sweep_tiles.split(",")], [int(x) for x in args.sizes.split(",")], log)
+        return 0
+    if not args.out:
+        ap.error("--out is required unless --sweep-tiles is given")
     h = sg.create()
     sg.load_plans(h, None)  # measure from the cost model's state, not an older table
     rows = []

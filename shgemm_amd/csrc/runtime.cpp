@@ -142,6 +142,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   if (st->forced_variant >= 0) {
     Plan pl = resolve_plan(st, tab, st->forced_variant, st->forced_slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
     pl.source = plan_forced;
+    apply_tile_sweep(st, pl, mode, oa, ob, oc);
     return pl;
   }
   // 2. kernel level fixed by the caller (reference shgemm.cu:655): no split-K

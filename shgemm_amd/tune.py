@@ -185,10 +185,18 @@ def _tiles(cls: int, mode: str, m: int, n: int) -> int:
 def sweep_tiles(candidates, sizes, log) -> None:
     """Times the compiled tile candidates of the op_n/op_n fp16 kernel
     (csrc/inst_tune.hip; MTK_SHGEMM_TUNE_P1=<i> is read when a handle is
-    created) at m = n = k = each size; -1 is the library's own plan."""
+    created and replaces the unsplit large-tile plan, which is forced here) at
+    m = n = k = each size; -1 is the library's own P1 tile."""
     import os
 
     torch = _torch()
+    # bring the GPU to its sustained clock before the first timed candidate
+    x = torch.randn(4096, 4096, device="cuda")
+    t_end = time.time() + 1.0
+    while time.time() < t_end:
+        x = x @ x
+        x /= x.norm()
+    torch.cuda.synchronize()
     for N in sizes:
         g = torch.Generator(device="cuda")
         g.manual_seed(1)
@@ -204,6 +212,7 @@ def sweep_tiles(candidates, sizes, log) -> None:
                 os.environ.pop("MTK_SHGEMM_TUNE_P1", None)
             os.environ["MTK_SHGEMM_EXP_SCALING"] = "0" if v == 0 else "1"  # candidate 0 is the two-accumulator mode
             h = sg.create()
+            sg.force_plan(h, 1, 1)  # the candidate replaces the unsplit large tile
             for k_, val in saved.items():
                 if val is None:
                     os.environ.pop(k_, None)

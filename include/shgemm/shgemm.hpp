@@ -137,12 +137,15 @@ void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 //   0 small tile (level P0), 1 large tile (P1), 2 128x128 split-K,
 //   3 large-tile split-K, 4 small-tile split-K, 5 256x256 tile of four
 //   128x128 wavefronts (one-accumulator modes; otherwise it runs as 1),
-//   6 large tile with the three-stage direct-to-LDS pipeline (prescaled fp16,
-//   op_b = op_n, aligned B, k a multiple of 32; otherwise it runs as 1).
+//   6 the 256x256 tile with a 3-slot direct-to-LDS ring for A and B
+//   (prescaled fp16, op_n/op_n, m and n multiples of 256, k of 32, aligned
+//   operands; otherwise it runs as 5), 7 large tile with the three-stage
+//   direct-to-LDS B pipeline (prescaled fp16, op_b = op_n, aligned B, k a
+//   multiple of 32; otherwise it runs as 1).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.
-enum { plan_variant_p2 = 5, plan_variant_bd = 6, num_plan_variants = 7 };
+enum { plan_variant_p2 = 5, plan_variant_p2r = 6, plan_variant_bd = 7, num_plan_variants = 8 };
 enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
 struct plan_t {
   int variant = 1;

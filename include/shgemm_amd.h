@@ -81,11 +81,13 @@ int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on);
  * plan table (written by shgemm_amd/tune.py) and a cost model.
  * Plan variants: 0 small tile (P0), 1 large tile (P1), 2 128x128 split-K,
  * 3 large-tile split-K, 4 small-tile split-K, 5 256x256 tile of four
- * 128x128 wavefronts, 6 large tile with the three-stage direct-to-LDS
- * pipeline (5 and 6 run as 1 where they do not apply).
+ * 128x128 wavefronts, 6 the same tile with a 3-slot direct-to-LDS ring for
+ * A and B, 7 large tile with the three-stage direct-to-LDS B pipeline
+ * (5-7 run as the nearest applicable variant where they do not apply).
  * Plan sources (query): 0 cost model, 1 plan table, 2 forced, 3 fixed level. */
 #define SHGEMM_AMD_PLAN_P2 5
-#define SHGEMM_AMD_PLAN_BD 6
+#define SHGEMM_AMD_PLAN_P2R 6
+#define SHGEMM_AMD_PLAN_BD 7
 /* Force variant/slices for every later call on h (variant < 0 clears). */
 int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
 /* Replace h's plan table with the file at path (NULL or "": no table). */

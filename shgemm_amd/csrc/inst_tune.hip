@@ -5,6 +5,7 @@ namespace shgemm_amd {
 template <class CFG, int MODE = MODE_FP16>
 static void put_tune(KernelTable& t) {
   static_assert(CFG::template lds_bytes<MODE>() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
+  if (t.ntune >= KernelTable::kMaxTune) return;  // list full: candidate not registered
   KernelEntry& e = t.tune[t.ntune++];
   e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OP_N, OP_N, OP_N, MODE, CFG, false>);
   e.lds = CFG::template lds_bytes<MODE>();
@@ -18,9 +19,24 @@ static void put_tune(KernelTable& t) {
 template <class CFG>
 static void put_tune_bd(KernelTable& t) {
   static_assert(BdLayout<CFG>::lds_bytes() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
+  if (t.ntune >= KernelTable::kMaxTune) return;  // list full: candidate not registered
   KernelEntry& e = t.tune[t.ntune++];
   e.fn = reinterpret_cast<const void*>(&shgemm_kernel_bd<OP_N, OP_N, CFG>);
   e.lds = BdLayout<CFG>::lds_bytes();
+  e.bm = CFG::BM;
+  e.bn = CFG::BN;
+  e.bk = CFG::BK;
+  e.mode = MODE_FP16S;
+  e.threads = CFG::NT;
+}
+// Ring-pipeline kernel (raw A and B straight to LDS rings; interior tiles).
+template <class CFG>
+static void put_tune_ring(KernelTable& t) {
+  static_assert(RingLayout<CFG>::lds_bytes() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
+  if (t.ntune >= KernelTable::kMaxTune) return;  // list full: candidate not registered
+  KernelEntry& e = t.tune[t.ntune++];
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ring<OP_N, CFG>);
+  e.lds = RingLayout<CFG>::lds_bytes();
   e.bm = CFG::BM;
   e.bn = CFG::BN;
   e.bk = CFG::BK;
@@ -45,5 +61,6 @@ void register_tune(KernelTable& t) {
   put_tune<TileCfg<192, 256, 32, 96, 128, 1, 1>, MODE_FP16S>(t);    // 13: 4 waves of 96x128
   put_tune<TileCfg<256, 192, 32, 128, 96, 1, 1>, MODE_FP16S>(t);    // 14: 4 waves of 128x96
   put_tune_bd<TileCfg<128, 256, 32, 64, 64, 1, 2>>(t);              // 15: deep prefetch, 8 waves of 64x64
+  put_tune_ring<TileCfg<256, 256, 32, 128, 128, 1, 1>>(t);          // 16: ring pipeline on the P2 tile
 }
 }  // namespace shgemm_amd

@@ -34,6 +34,20 @@ inline void put_grouped(KernelTable& t, int cls) {
   e.threads = CFG::NT;
 }
 
+template <int OPC, class CFG>
+inline void put_ring(KernelTable& t) {
+  static_assert(RingLayout<CFG>::lds_bytes() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
+  KernelEntry& e = t.shgemm[OP_N][OP_N][OPC][MODE_FP16S][V_P2R];
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ring<OPC, CFG>);
+  e.lds = RingLayout<CFG>::lds_bytes();
+  e.bm = CFG::BM;
+  e.bn = CFG::BN;
+  e.bk = CFG::BK;
+  e.mode = MODE_FP16S;
+  e.threads = CFG::NT;
+  e.wgs_per_cu = 1;
+}
+
 template <int OPA, int OPB, int OPC, int MODE>
 inline void put_shgemm_opc(KernelTable& t) {
   using G = GroupCfg<MODE>;
@@ -47,6 +61,7 @@ inline void put_shgemm_opc(KernelTable& t) {
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, true>(t, V_P1SK);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, true>(t, V_P0SK);
   if constexpr (C::kHasP2) put_shgemm<OPA, OPB, OPC, MODE, typename C::P2, false>(t, V_P2);
+  if constexpr (MODE == MODE_FP16S && OPA == OP_N && OPB == OP_N) put_ring<OPC, typename C::P2>(t);
 }
 
 template <int OPA, int OPB, int MODE>

@@ -29,7 +29,8 @@ enum : int {
   V_P1SK = 3,  // large tile, split-K
   V_P0SK = 4,  // small tile, split-K
   V_P2 = 5,    // 256 x 256 tile of four 128 x 128 wavefronts (one per SIMD)
-  kNumVariants = 6
+  V_P2R = 6,   // the same tile with the 3-slot direct-to-LDS ring (prescaled fp16, op_n/op_n, whole tiles)
+  kNumVariants = 7
 };
 inline bool variant_is_split(int v) { return v == V_SK || v == V_P1SK || v == V_P0SK; }
 inline int variant_level(int v) { return (v == V_P0 || v == V_P0SK) ? LEVEL_P0 : LEVEL_P1; }
@@ -48,7 +49,7 @@ struct KernelTable {
   KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,
   // timed by the tile sweep and selectable with MTK_SHGEMM_TUNE_P1=<index>.
-  static constexpr int kMaxTune = 16;
+  static constexpr int kMaxTune = 24;
   KernelEntry tune[kMaxTune];
   int ntune = 0;
 };

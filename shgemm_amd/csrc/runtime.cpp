@@ -99,14 +99,26 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
-// A plan variant 0..6 with a slice count -> a launchable plan.  Split
+// The ring kernel runs whole 256 x 256 tiles and whole k-tiles (at least
+// two), with vector-aligned operands whose tile offsets fit 32 bits.
+bool ring_applies(const shgemm_amd::KernelEntry& e, int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b,
+                  int64_t lda, int64_t ldb) {
+  return e.fn != nullptr && vec_a && vec_b && m % e.bm == 0 && n % e.bn == 0 && k % e.bk == 0 && k >= 2 * e.bk &&
+         (int64_t)e.bk * lda * 4 < (int64_t(1) << 32) && (int64_t)e.bn * ldb * 2 < (int64_t(1) << 32);
+}
+
+// A plan variant 0..7 with a slice count -> a launchable plan.  Split
 // variants get 2..max_slices slices; when k or the workspace allows no split
-// they run as the unsplit variant of the same tile.
+// they run as the unsplit variant of the same tile.  Variants that do not
+// apply to the arguments run as the nearest one that does.
 Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants],
                   int requested, int slices, int mode, int oa, int ob, int oc, int64_t m, int64_t n, int64_t k,
-                  bool vec_a, bool vec_b) {
+                  bool vec_a, bool vec_b, int64_t lda, int64_t ldb) {
   Plan pl;
   pl.requested = requested;
+  if (requested == shgemm_amd::V_P2R &&
+      !ring_applies(tab[shgemm_amd::V_P2R], m, n, k, vec_a, vec_b, lda, ldb))
+    requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
   if (requested == plan_variant_bd) {
     pl.variant = shgemm_amd::V_P1;
     pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
@@ -137,10 +149,11 @@ bool apply_tile_sweep(const detail::handle_state* st, Plan& pl, int mode, int oa
 
 Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
                  const shgemm_amd::KernelEntry (&tab)[shgemm_amd::kNumVariants], int mode, int oa, int ob, int oc,
-                 int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b) {
+                 int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b, int64_t lda, int64_t ldb) {
   // 1. explicit override (measurement and tuning)
   if (st->forced_variant >= 0) {
-    Plan pl = resolve_plan(st, tab, st->forced_variant, st->forced_slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
+    Plan pl = resolve_plan(st, tab, st->forced_variant, st->forced_slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b,
+                           lda, ldb);
     pl.source = plan_forced;
     apply_tile_sweep(st, pl, mode, oa, ob, oc);
     return pl;
@@ -157,7 +170,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   // 3. measured plan table
   if (const shgemm_amd::PlanRow* r =
           shgemm_amd::find_plan(st->plans, shgemm_amd::PLAN_SINGLE, mode, oa, ob, oc, m, n, k)) {
-    Plan pl = resolve_plan(st, tab, r->choice, r->slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b);
+    Plan pl = resolve_plan(st, tab, r->choice, r->slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b, lda, ldb);
     pl.source = plan_table;
     apply_tile_sweep(st, pl, mode, oa, ob, oc);
     return pl;
@@ -168,7 +181,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   double best_t = 1e30;
   for (int v = 0; v < shgemm_amd::kNumVariants; ++v) {
     const auto& e = tab[v];
-    if (e.fn == nullptr) continue;
+    if (e.fn == nullptr || v == shgemm_amd::V_P2R) continue;  // the ring variant is chosen by measurement only
     const bool split = shgemm_amd::variant_is_split(v);
     const int64_t smax = split ? max_slices(e, m, n, k, st->ws_bytes) : 1;
     for (int64_t s = split ? 2 : 1; s <= smax; s *= 2) {
@@ -339,8 +352,10 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
   const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int oc = op_c == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
+  // aligned operands with the smallest leading dimensions
+  const int64_t lda = (int64_t)(op_a == op_n ? m : k), ldb = (int64_t)(op_b == op_n ? k : n);
   const Plan pl = choose_plan(handle, st, st->table.shgemm[oa][ob][oc][mode], mode, oa, ob, oc, (int64_t)m, (int64_t)n,
-                              (int64_t)k, true, true);
+                              (int64_t)k, true, true, lda, ldb);
   plan_t out;
   out.variant = pl.entry != nullptr && pl.requested == plan_variant_bd ? plan_variant_bd : pl.variant;
   out.slices = pl.slices;
@@ -413,7 +428,7 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
 
   const auto& tab = st->table.shgemm[oa][ob][oc][mode];
   const Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
-                                p.vec_b != 0);
+                                p.vec_b != 0, p.lda, p.ldb);
   const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);

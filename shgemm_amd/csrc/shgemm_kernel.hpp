@@ -694,6 +694,164 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmPar
 }
 
 // ---------------------------------------------------------------------------
+// m-contiguous FP32 tile (A with op_n) straight into LDS, raw: BK columns of
+// BM = 256 rows, column c at slots [c*64, c*64 + 64) (element (r, c) at float
+// c*BM + r).  One wavefront instruction moves one column: lane l fetches rows
+// 4l..4l+3.  Wavefront w issues columns w, w + WAVES, ...  The tile base is
+// wave-uniform; the caller guarantees whole columns in range, a 16-byte
+// aligned base with lda % 4 == 0, and BK*lda*4 below 2^32.
+// ---------------------------------------------------------------------------
+template <int BM, int BK, int NT>
+struct DirectRawF32 {
+  static constexpr int QPC = BM / 4;
+  static constexpr int WAVES = NT / 64;
+  static constexpr int PER = BK / WAVES;  // instructions per thread per k-tile
+  static_assert(QPC == 64, "one wavefront instruction per column");
+  static_assert(BK % WAVES == 0, "columns split evenly over the wavefronts");
+  __device__ __forceinline__ static void load(const float* __restrict__ base, int64_t ld, int m0, int k0,
+                                              uint4* __restrict__ dst, int tid) {
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const char* tile = uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)k0 * ld + m0));
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int c = q * WAVES + wave;
+      const uint32_t boff = 4u * ((uint32_t)c * (uint32_t)ld + 4u * (uint32_t)lane);
+      __builtin_amdgcn_global_load_lds(tile + boff, reinterpret_cast<uint32_t*>(dst + c * QPC), 16, 0, 0);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Ring pipeline of the prescaled fp16 P2 tile (256 x 256 x 32, four
+// wavefronts of 128 x 128) for op_a = op_b = op_n, interior tiles, whole
+// k-tiles.  Each k-tile's raw FP32 A and FP16 B go straight to LDS into a
+// 3-slot ring, issued two k-tiles ahead of their MFMA work, so a load has
+// about two k-tiles of MFMA time to land and holds no registers in flight.
+// Each wavefront reads its own A fragments as FP32 and splits them in
+// registers just before the MFMAs that use them (the same LDS bytes as
+// reading a stored hi/lo pair; the split work interleaves with the MFMAs);
+// one barrier per k-tile.  Every thread issues the same loads each k-tile,
+// so the waits are exact vmcnt counts.  The k order of each C element is the
+// register-staged kernel's, so the results are bit-identical to it.
+// LDS per slot (16-byte slots): raw A (BK*BM/4) + B (BN*KC); 3 slots = 144 KiB.
+// ---------------------------------------------------------------------------
+template <class CFG>
+struct RingLayout {
+  static constexpr int RAW = CFG::BK * CFG::BM / 4;
+  static constexpr int BST = CFG::BN * CFG::KC;
+  static constexpr int SLOT = RAW + BST;
+  static constexpr int NSLOT = 3;
+  static constexpr size_t lds_bytes() { return (size_t)NSLOT * SLOT * 16; }
+};
+
+template <int OPC, class CFG>
+__device__ __forceinline__ void shgemm_tile_ring(const GemmParams& p, int m0, int n0, uint4* __restrict__ lds) {
+  constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM;
+  constexpr bool LANE_M = (OPC == OP_N);
+  using L = RingLayout<CFG>;
+  using DA = DirectRawF32<BM, BK, NT>;
+  using DB = DirectF16<BN, BK, NT>;
+  constexpr int PER = DA::PER + DB::PER;  // vector-memory instructions per thread per k-tile
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int lrow = lane & 31, lh = lane >> 5;
+  const int ktiles = p.k / BK;
+  const float* __restrict__ A = p.a;
+  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+  float sc[FM];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+    sc[i] = p.row_exp != nullptr ? __builtin_ldexpf(1.0f, -p.row_exp[m0 + wm * WM + i * 32 + lrow]) : 1.0f;
+
+  f32x16 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  auto issue = [&](int kt) {
+    uint4* slot = lds + (kt % L::NSLOT) * L::SLOT;
+    DB::load(B, p.ldb, n0, kt * BK, p.n, slot + L::RAW, tid);
+    DA::load(A, p.lda, m0, kt * BK, slot, tid);
+  };
+  auto compute = [&](int kt) {
+    const uint4* slot = lds + (kt % L::NSLOT) * L::SLOT;
+    const float* rf = reinterpret_cast<const float*>(slot);  // element (r, k) at rf[k*BM + r]
+    const uint4* bst = slot + L::RAW;
+#pragma unroll
+    for (int s = 0; s < BK / 16; ++s) {
+      const int c = 2 * s + lh;
+      f16x8 ah[FM], al[FM], bb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * WM + i * 32 + lrow;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          _Float16 hh, ll;
+          split_f16_same_scale(rf[(8 * c + e) * BM + r] * sc[i], hh, ll);
+          ah[i][e] = hh;
+          al[i][e] = ll;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if constexpr (LANE_M) {
+            acc[i][j] = mfma_f16(bb[j], ah[i], acc[i][j]);
+            acc[i][j] = mfma_f16(bb[j], al[i], acc[i][j]);
+          } else {
+            acc[i][j] = mfma_f16(ah[i], bb[j], acc[i][j]);
+            acc[i][j] = mfma_f16(al[i], bb[j], acc[i][j]);
+          }
+        }
+    }
+  };
+
+  issue(0);
+  issue(1);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));  // tile 0 landed (tile 1 may be in flight)
+  __syncthreads();
+  for (int kt = 0; kt < ktiles; ++kt) {
+    // the slot of tile kt+2 held tile kt-1, whose reads ended at the last barrier
+    if (kt + 2 < ktiles) issue(kt + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(kt);
+    __builtin_amdgcn_sched_barrier(0);
+    // tile kt+1 landed once at most tile kt+2's loads are outstanding
+    if (kt + 2 < ktiles) __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    __syncthreads();
+  }
+  f32x16 out[1][FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) out[0][i][j] = acc[i][j];
+  store_ctile<MODE_FP16S, LANE_M, false, 1, FM, FN, WM, WN>(out, p, m0, n0, wm, wn, lrow, lh, nullptr);
+}
+
+// The ring kernel runs only whole tiles: the host launches it when m and n
+// are multiples of the tile, k is a multiple of BK with at least two k-tiles,
+// A is 16-byte aligned with lda % 4 == 0, B with ldb % 8 == 0, and the tile
+// offsets fit the 32-bit forms (BK*lda*4 and BN*ldb*2 below 2^32).
+template <int OPC, class CFG>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ring(const GemmParams p) {
+  extern __shared__ uint4 lds[];
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  shgemm_tile_ring<OPC, CFG>(p, tm * CFG::BM, tn * CFG::BN, lds);
+}
+
+// ---------------------------------------------------------------------------
 // Deep-prefetch variant of the prescaled fp16 kernel for op_b = op_n with
 // full k-tiles (k % BK == 0) and aligned B: B goes straight to LDS into a
 // 3-stage ring and A through two register sets, both issued two k-tiles

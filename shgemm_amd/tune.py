@@ -7,8 +7,8 @@ measurements on the device at hand.
 
 Two kinds of rows (format in csrc/plans.hpp):
   single   one shgemm call at shape point (2^lm, 2^ln, 2^lk): every plan
-           variant (P0, P1, P2, P1 three-stage, and the split-K variants at
-           2..256 slices) is timed alone; the fastest is kept.
+           variant (P0, P1, P2, P2 ring, P1 three-stage, and the split-K
+           variants at 2..256 slices) is timed alone; the fastest is kept.
   grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
            2 large) x k-slices {1, 2, 4} are timed on a group of identical
            copies large enough to fill the GPU several times over, so the
@@ -111,7 +111,8 @@ def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
         B = torch.randn(k * n, device="cuda").half()
         C = torch.empty(m * n, device="cuda")
         seen, best = set(), None
-        candidates = [(0, 1), (1, 1), (5, 1), (6, 1)] + [(v, s) for v in (2, 3, 4) for s in (2, 4, 8, 16, 32, 64, 128, 256)]
+        candidates = [(0, 1), (1, 1), (5, 1), (6, 1), (7, 1)] + [(v, s) for v in (2, 3, 4)
+                                                                  for s in (2, 4, 8, 16, 32, 64, 128, 256)]
         for v, s in candidates:
             sg.force_plan(h, v, s)
             eff = sg.query_plan(h, oa, ob, m, n, k, ct, oc)[:2]
@@ -265,7 +266,7 @@ def main(argv=None) -> int:
         f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}, {props.multi_processor_count} CUs)",
         f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
         f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
-        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P1 three-stage);"
+        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage);"
         " time_us per call",
         "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large); time_us per problem in a full group",
     ]

@@ -72,7 +72,7 @@ def test_bad_plan_table_is_rejected(h, tmp_path):
         sg.load_plans(h, bad)
     assert ei.value.status == sg.STATUS_EINVAL and "bad.tsv:1" in str(ei.value)
     with pytest.raises(sg.ShgemmError):
-        sg.force_plan(h, 7, 1)
+        sg.force_plan(h, 8, 1)
 
 
 def test_table_plan_runs_named_kernel(h, tmp_path):
@@ -117,15 +117,16 @@ def test_shipped_table_loads_at_create(h):
 @pytest.mark.parametrize("ct,scaling", [(0, True), (1, True), (0, False)])
 @pytest.mark.parametrize("ops", [(0, 0, 0), (1, 0, 1), (0, 1, 0), (1, 1, 1)])
 def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
-    """The 256x256 four-wavefront tile (plan 5), the large tile (plan 1) and
-    the three-stage pipeline (plan 6, where it applies) accumulate every C
+    """The 256x256 four-wavefront tile (plan 5), its direct-to-LDS ring form
+    (plan 6, on whole tiles: the 512x768x256 case), the large tile (plan 1)
+    and the three-stage pipeline (plan 7, where it applies) accumulate every C
     element over k in the same order, so unsplit results are bit-identical;
     odd sizes exercise every edge path."""
     import shgemm_amd as sg
 
     op_a, op_b, op_c = ops
     rng = np.random.default_rng(100 + 8 * ct + 4 * op_a + 2 * op_b + op_c)
-    for m, n, k in [(300, 517, 333), (1000, 600, 704), (257, 255, 96)]:
+    for m, n, k in [(300, 517, 333), (1000, 600, 704), (257, 255, 96), (512, 768, 256)]:
         lda, ldb, ldc = (m if op_a == 0 else k) + 1, (k if op_b == 0 else n) + 2, (m if op_c == 0 else n) + 3
         A = oracle.gen_uniform(rng, lda * (k if op_a == 0 else m))
         B16 = oracle.gen_f16_normal(rng, ldb * (n if op_b == 0 else k))
@@ -134,7 +135,7 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
         c_t, c_p = dev_buffer(np.zeros(ldc * (n if op_c == 0 else m), np.float32))
         sg.set_exponent_scaling(h, scaling)
         outs = []
-        for plan in (1, 5, 6):
+        for plan in (1, 5, 6, 7):
             sg.force_plan(h, plan, 1)
             sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, ct, op_c)
             outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())

@@ -31,10 +31,11 @@ PEAK_HBM = 8.0             # TB/s
 
 def parse():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["square", "grouped", "serve", "configs"], default="square",
+    ap.add_argument("--workload", choices=["square", "grouped", "serve", "configs", "bands", "energy"], default="square",
                     help="square: m=n=k=--size (the headline line); grouped: 512 uneven problems in one group; "
                          "serve: sketch server under Poisson arrivals; configs: the 1024^3, sketch and "
-                         "tall-skinny configurations, one JSON line each")
+                         "tall-skinny configurations, one JSON line each; bands: error by exponent band of A; "
+                         "energy: W and J per GEMM from the card's energy counter")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -47,7 +48,7 @@ def parse():
     ap.add_argument("--problems", type=int, default=512, help="grouped workload: number of problems")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--rates", default="200,500,1000,2000", help="serve: offered request rates (req/s)")
-    ap.add_argument("--seconds", type=float, default=3.0, help="serve: arrival window per rate")
+    ap.add_argument("--seconds", type=float, default=3.0, help="serve: arrival window per rate; energy: measuring window per GEMM")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent lanes (handle + stream each)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget")
     return ap.parse_args()
@@ -456,6 +457,128 @@ def bench_configs(args):
     sg.destroy(h)
 
 
+def bench_bands(args):
+    """Accuracy and speed by exponent band of A (the reference's exponent
+    study): A = 2^(u*(hi-lo)+lo), u ~ U[0,1), bands [-15,14], [-36,-15],
+    [-100,-36]; B = FP16(U[0,1)); m = n = k = 4096 op_n/op_n.  Per band and
+    compute type: relative Frobenius and max relative error against FP64 on
+    256 sampled rows, beside the library SGEMM's, and TFLOP/s."""
+    import numpy as np
+    import torch
+
+    import shgemm_amd as sg
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    h = sg.create()
+    sg.set_cuda_stream(h, stream)
+    m = n = k = 4096
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    B = torch.rand(k * n, device=dev, generator=g).half()
+    B32 = B.float()
+    Bh = B32.view(n, k).t().cpu().numpy().astype(np.float64)
+    C = torch.empty(m * n, device=dev)
+    rows = np.sort(np.random.default_rng(3).choice(m, 256, replace=False))
+    ridx = torch.from_numpy(rows).to(dev)
+    for lo, hi in [(-15, 14), (-36, -15), (-100, -36)]:
+        A = torch.exp2(torch.rand(m * k, device=dev, generator=g) * (hi - lo) + lo)
+        Ar = A.view(k, m).t()[ridx].cpu().numpy().astype(np.float64)
+        ref = Ar @ Bh
+
+        def errs():
+            torch.cuda.synchronize()
+            got = C.view(n, m).t()[ridx].cpu().numpy().astype(np.float64)
+            rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            mx = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
+            return rel, mx
+
+        sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, C, m)
+        s_rel, s_max = errs()
+        for name, ct, scaling in (("fp16", sg.fp16, True), ("fp16 classic", sg.fp16, False), ("tf32", sg.tf32, True)):
+            sg.set_exponent_scaling(h, scaling)
+            call = lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, ct)
+            ms = _time_ms(call, 5, stream)
+            call()
+            rel, mx = errs()
+            print(json.dumps({"band": f"{lo}:{hi}", "compute_type": name, "m": m, "n": n, "k": k,
+                              "rel_fro": rel, "max_rel": mx, "sgemm_rel_fro": s_rel, "sgemm_max_rel": s_max,
+                              "ratio": rel / s_rel, "tflops_eff": round(2.0 * m * n * k / (ms * 1e-3) / 1e12, 2)}),
+                  flush=True)
+        sg.set_exponent_scaling(h, True)
+        del A
+    sg.destroy(h)
+
+
+def _energy_counter():
+    """Card energy reader from the ROCm SMI library: returns a function giving
+    the accumulated energy in joules (accumulator x resolution, in uJ)."""
+    try:
+        import amdsmi
+    except ImportError as e:  # the energy workload needs the SMI library
+        raise SystemExit(f"energy workload: amdsmi is not importable ({e})")
+    amdsmi.amdsmi_init()
+    dev = amdsmi.amdsmi_get_processor_handles()[0]
+
+    def joules():
+        r = amdsmi.amdsmi_get_energy_count(dev)
+        return r["energy_accumulator"] * r["counter_resolution"] * 1e-6
+
+    return joules
+
+
+def bench_energy(args):
+    """Power and energy per GEMM (the reference's power study,
+    test/power_consumption_test.cu:55-126, 230-301): m = n = k = 2^10..2^14,
+    op_n/op_n, A U[0,1), B FP16(U[0,1)); for tf32, fp16 and the library SGEMM,
+    time 16 calls, then repeat the call for about --seconds and read the
+    card's energy counter around that window.  Prints the reference's columns
+    (throughput, average W, J per GEMM, call count) plus GFLOP/J."""
+    import torch
+
+    import shgemm_amd as sg
+
+    joules = _energy_counter()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    h = sg.create()
+    sg.set_cuda_stream(h, stream)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    top = 1 << 14
+    A = torch.rand(top * top, device=dev, generator=g)
+    B = torch.rand(top * top, device=dev, generator=g).half()
+    B32 = B.float()
+    C = torch.empty(top * top, device=dev)
+    window = args.seconds
+    for log_n in range(10, 15):
+        m = n = k = 1 << log_n
+        calls = [("tf32", lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.tf32)),
+                 ("fp16", lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16)),
+                 ("sgemm_fp32", lambda: sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, C, m))]
+        for name, fn in calls:
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(16):
+                fn()
+            torch.cuda.synchronize()
+            per_call = (time.perf_counter() - t0) / 16
+            count = max(1, int(window / per_call))
+            e0, t0 = joules(), time.perf_counter()
+            for _ in range(count):
+                fn()
+            torch.cuda.synchronize()
+            e1, t1 = joules(), time.perf_counter()
+            flop = 2.0 * m * n * k
+            print(json.dumps({"compute_type": name, "m": m, "n": n, "k": k, "op_a": "N", "op_b": "N",
+                              "throughput_tflops": round(flop / per_call / 1e12, 2),
+                              "avg_power_w": round((e1 - e0) / (t1 - t0), 1),
+                              "energy_per_gemm_j": (e1 - e0) / count, "test_count": count,
+                              "gflop_per_j": round(flop * count / (e1 - e0) / 1e9, 1)}), flush=True)
+    sg.destroy(h)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -470,6 +593,12 @@ def main():
         return
     if args.workload == "configs":
         bench_configs(args)
+        return
+    if args.workload == "bands":
+        bench_bands(args)
+        return
+    if args.workload == "energy":
+        bench_energy(args)
         return
 
     import numpy as np

@@ -97,6 +97,22 @@ __device__ __forceinline__ void split_f16_same_scale(float x, _Float16& hi, _Flo
   lo = (_Float16)(x - (float)hi);
 }
 
+// Two elements at once, scale s = 2^-e applied inside: hi = FP16(x*s),
+// lo = FP16(x*s - hi), packed (element 0 in bits 15:0).  Each mixed-precision
+// FMA forms x*s (+ -hi) exactly and rounds once to FP16, so the bits equal
+// split_f16_same_scale(x*s) (x*s is exact in FP32: s is a power of two that
+// keeps the row's largest element below 2^15).  Four VALU instructions per
+// pair, where the C form is compiled to seven (hi is formed twice, once
+// packed and once for the residual).
+__device__ __forceinline__ void split2_f16_scaled(float x0, float x1, float s, uint32_t& hi, uint32_t& lo) {
+  asm("v_fma_mixlo_f16 %0, %2, %4, 0\n\t"
+      "v_fma_mixhi_f16 %0, %3, %4, 0\n\t"
+      "v_fma_mixlo_f16 %1, %2, %4, -%0 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %1, %3, %4, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+      : "=&v"(hi), "=&v"(lo)
+      : "v"(x0), "v"(x1), "v"(s));
+}
+
 // FP32 -> three BF16 parts a0 + a1 + a2 (each residual exact in FP32).
 __device__ __forceinline__ void split_bf16x3(float a, __bf16& a0, __bf16& a1, __bf16& a2) {
   a0 = (__bf16)a;

@@ -62,5 +62,7 @@ void register_tune(KernelTable& t) {
   put_tune<TileCfg<256, 192, 32, 128, 96, 1, 1>, MODE_FP16S>(t);    // 14: 4 waves of 128x96
   put_tune_bd<TileCfg<128, 256, 32, 64, 64, 1, 2>>(t);              // 15: deep prefetch, 8 waves of 64x64
   put_tune_ring<TileCfg<256, 256, 32, 128, 128, 1, 1>>(t);          // 16: ring pipeline on the P2 tile
+  put_tune<TileCfg<256, 256, 32, 128, 128, 2, 1>, MODE_FP16S>(t);   // 17: P2, two k-tiles of register prefetch
+  put_tune<TileCfg<256, 256, 32, 128, 128, 2, 1, true>, MODE_FP16S>(t);  // 18: 17 with the skewed barrier (the library's P2)
 }
 }  // namespace shgemm_amd

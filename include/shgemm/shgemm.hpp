@@ -160,6 +160,15 @@ void force_plan(shgemmHandle_t& handle, const int variant, const int slices);
 // leaves the handle with no table (cost model only).  create() loads
 // $MTK_SHGEMM_PLANS, else plans_gfx950.tsv beside the library, if present.
 void load_plans(shgemmHandle_t& handle, const char* path);
+
+// Kernel timing (no counterpart in the reference; for profiling and
+// benchmarks): HIP events on the handle's stream bracket the main GEMM
+// kernel of each of the next `max_launches` shgemm / hsgemm / sgemm_fp32
+// calls (not the row-exponent pass or the split-K reduction); 0 stops.
+void enable_kernel_timing(shgemmHandle_t& handle, const int max_launches);
+// Waits for the bracketed launches since the last query and returns their
+// count and summed kernel time; the event pairs are then free again.
+void kernel_timing(shgemmHandle_t& handle, int& launches, float& total_ms);
 // The plan shgemm would use for these arguments (nothing is launched).
 plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
                   const std::size_t m, const std::size_t n, const std::size_t k, const tc_t compute_type,

@@ -90,6 +90,12 @@ int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on);
 #define SHGEMM_AMD_PLAN_BD 7
 /* Force variant/slices for every later call on h (variant < 0 clears). */
 int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
+/* Kernel timing (no counterpart in the reference; for profiling): HIP events
+ * on h's stream around the main GEMM kernel of the next max_launches calls
+ * (0 stops).  shgemm_amd_kernel_timing waits for them and returns the count
+ * and the summed milliseconds since the last query. */
+int shgemm_amd_enable_kernel_timing(shgemm_amd_handle* h, int max_launches);
+int shgemm_amd_kernel_timing(shgemm_amd_handle* h, int* launches, float* total_ms);
 /* Replace h's plan table with the file at path (NULL or "": no table). */
 int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path);
 /* The plan shgemm would run for these arguments, for aligned operands. */

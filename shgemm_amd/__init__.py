@@ -78,6 +78,8 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_hsgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
     L.shgemm_amd_sgemm_fp32.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32]
     L.shgemm_amd_set_exponent_scaling.argtypes = [vp, i32]
+    L.shgemm_amd_enable_kernel_timing.argtypes = [vp, i32]
+    L.shgemm_amd_kernel_timing.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_float)]
     L.shgemm_amd_force_plan.argtypes = [vp, i32, i32]
     L.shgemm_amd_load_plans.argtypes = [vp, ctypes.c_char_p]
     L.shgemm_amd_query_plan.argtypes = [vp, i32, i32, sz, sz, sz, i32, i32, ip, ip, ip]
@@ -221,6 +223,20 @@ PLAN_SOURCES = {0: "cost model", 1: "table", 2: "forced", 3: "fixed level"}
 def force_plan(handle, variant: int, slices: int = 1) -> None:
     """Run every later call on `handle` with this plan (variant < 0 clears)."""
     _check(load_library().shgemm_amd_force_plan(handle.ptr, int(variant), int(slices)))
+
+
+def enable_kernel_timing(handle, max_launches: int) -> None:
+    """Bracket the main GEMM kernel of the next `max_launches` calls on
+    `handle` with HIP events on its stream (0 stops)."""
+    _check(load_library().shgemm_amd_enable_kernel_timing(handle.ptr, int(max_launches)))
+
+
+def kernel_timing(handle) -> tuple[int, float]:
+    """(launches, total kernel ms) of the bracketed launches since the last query."""
+    n = ctypes.c_int(0)
+    ms = ctypes.c_float(0.0)
+    _check(load_library().shgemm_amd_kernel_timing(handle.ptr, ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, float(ms.value)
 
 
 def load_plans(handle, path) -> None:

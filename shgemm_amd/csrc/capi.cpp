@@ -164,6 +164,24 @@ int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices) {
   });
 }
 
+int shgemm_amd_enable_kernel_timing(shgemm_amd_handle* h, int max_launches) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::enable_kernel_timing(h->h, max_launches);
+  });
+}
+
+int shgemm_amd_kernel_timing(shgemm_amd_handle* h, int* launches, float* total_ms) {
+  return guarded([&] {
+    need(h);
+    int n = 0;
+    float ms = 0.0f;
+    mtk::shgemm::kernel_timing(h->h, n, ms);
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+  });
+}
+
 int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path) {
   return guarded([&] {
     need(h);

@@ -43,6 +43,28 @@ void launch(const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmP
   check_hip(hipLaunchKernel(e.fn, grid, dim3(e.threads), args, e.lds, s), "kernel launch");
 }
 
+// The main GEMM kernel of a call, bracketed by the handle's timing events
+// while enable_kernel_timing has free event pairs.
+void launch_main(detail::handle_state* st, const shgemm_amd::KernelEntry& e, dim3 grid, const shgemm_amd::GemmParams& p,
+                 hipStream_t s) {
+  const bool timed = st->timing_used < (int)st->timing.size();
+  if (timed) check_hip(hipEventRecord(st->timing[st->timing_used].first, s), "timing event");
+  launch(e, grid, p, s);
+  if (timed) {
+    check_hip(hipEventRecord(st->timing[st->timing_used].second, s), "timing event");
+    ++st->timing_used;
+  }
+}
+
+void release_timing(detail::handle_state* st) {
+  for (auto& ev : st->timing) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  st->timing.clear();
+  st->timing_used = 0;
+}
+
 const char* op_name(operation_t op) { return op == op_n ? "N" : "T"; }
 
 // A kernel choice for one call: table variant (or an explicit entry) and the
@@ -302,6 +324,7 @@ void destroy(shgemmHandle_t& handle) {
   if (st->ws) (void)hipFree(st->ws);
   if (st->row_max) (void)hipFree(st->row_max);
   if (st->row_exp) (void)hipFree(st->row_exp);
+  release_timing(st);
   delete st;
   handle.state = nullptr;
 }
@@ -445,12 +468,12 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
     const int64_t used = ((int64_t)k + kslice - 1) / kslice;
     p.k_slice = (int)kslice;
     p.ws = st->ws;
-    launch(e, dim3((unsigned)tiles, (unsigned)used), p, stream);
+    launch_main(st, e, dim3((unsigned)tiles, (unsigned)used), p, stream);
     check_hip(shgemm_amd::launch_splitk_reduce(st->ws, (int)used, elems, inner, alpha, beta, c_ptr, (int64_t)ldc,
                                                stream),
               "split-K reduction");
   } else {
-    launch(e, dim3((unsigned)tiles), p, stream);
+    launch_main(st, e, dim3((unsigned)tiles), p, stream);
   }
   return static_cast<detail::kernel_level>(plan.level);
 }
@@ -496,7 +519,38 @@ void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const opera
   if (handle.debug_mode)
     std::printf("[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s] sgemm_fp32 tile=%dx%dx%d\n", m, n, k, op_name(op_a),
                 op_name(op_b), e.bm, e.bn, e.bk);
-  launch(e, dim3((unsigned)tiles), p, stream);
+  launch_main(handle.state, e, dim3((unsigned)tiles), p, stream);
+}
+
+void enable_kernel_timing(shgemmHandle_t& handle, const int max_launches) {
+  if (handle.state == nullptr) throw error("handle not created");
+  if (max_launches < 0 || max_launches > 4096) throw error("max_launches must be in 0..4096");
+  auto* st = handle.state;
+  if (!st->timing.empty()) (void)hipStreamSynchronize(handle.cuda_stream);  // events may still be pending
+  release_timing(st);
+  for (int i = 0; i < max_launches; ++i) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      if (a) (void)hipEventDestroy(a);
+      release_timing(st);
+      throw shgemm_amd::hip_failure("timing event creation failed");
+    }
+    st->timing.emplace_back(a, b);
+  }
+}
+
+void kernel_timing(shgemmHandle_t& handle, int& launches, float& total_ms) {
+  if (handle.state == nullptr) throw error("handle not created");
+  auto* st = handle.state;
+  launches = st->timing_used;
+  total_ms = 0.0f;
+  for (int i = 0; i < st->timing_used; ++i) {
+    check_hip(hipEventSynchronize(st->timing[i].second), "timing event wait");
+    float ms = 0.0f;
+    check_hip(hipEventElapsedTime(&ms, st->timing[i].first, st->timing[i].second), "timing event");
+    total_ms += ms;
+  }
+  st->timing_used = 0;
 }
 
 }  // namespace shgemm

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/shgemm/shgemm.hpp"
@@ -31,6 +32,9 @@ struct handle_state {
   int forced_slices = 1;
   std::vector<shgemm_amd::PlanRow> plans;  // measured plan table
   std::string plans_path;
+  // enable_kernel_timing: event pairs recorded around the main GEMM kernel
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timing;
+  int timing_used = 0;
 };
 
 }  // namespace detail

@@ -143,3 +143,32 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
         sg.set_exponent_scaling(h, True)
         for o in outs[1:]:
             assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32)), (m, n, k)
+
+
+def test_kernel_timing_counts_main_kernel_launches(h):
+    """enable_kernel_timing brackets the main GEMM kernel of the next N calls;
+    each query returns the launches since the previous one."""
+    import shgemm_amd as sg
+
+    m = n = k = 512
+    rng = np.random.default_rng(5)
+    a_t, a_p = dev_buffer(oracle.gen_uniform(rng, m * k))
+    b_t, b_p = dev_buffer(oracle.gen_f16_normal(rng, k * n))
+    c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
+    call = lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, sg.fp16)
+    sg.enable_kernel_timing(h, 3)
+    for _ in range(5):
+        call()
+    launches, ms = sg.kernel_timing(h)
+    assert launches == 3 and 0.0 < ms < 1e4
+    call()  # the pairs are free again after a query
+    assert sg.kernel_timing(h)[0] == 1
+    sg.force_plan(h, 2, 4)  # split-K: the main kernel is timed, the reduction is not
+    call()
+    assert sg.kernel_timing(h)[0] == 1
+    sg.force_plan(h, -1, 1)
+    sg.enable_kernel_timing(h, 0)
+    call()
+    assert sg.kernel_timing(h) == (0, 0.0)
+    with pytest.raises(sg.ShgemmError):
+        sg.enable_kernel_timing(h, -1)

@@ -514,8 +514,9 @@ def bench_bands(args):
 
 
 def _energy_counter():
-    """Card energy reader from the ROCm SMI library: returns a function giving
-    the accumulated energy in joules (accumulator x resolution, in uJ)."""
+    """Card readers from the ROCm SMI library: returns (joules(), sample()) --
+    the accumulated energy in joules (accumulator x resolution, in uJ) and
+    one (GFX clock MHz, socket power W) sample from the GPU metrics table."""
     try:
         import amdsmi
     except ImportError as e:  # the energy workload needs the SMI library
@@ -527,7 +528,43 @@ def _energy_counter():
         r = amdsmi.amdsmi_get_energy_count(dev)
         return r["energy_accumulator"] * r["counter_resolution"] * 1e-6
 
-    return joules
+    def sample():
+        m = amdsmi.amdsmi_get_gpu_metrics_info(dev)
+        return m.get("current_gfxclk"), m.get("current_socket_power")
+
+    return joules, sample
+
+
+class _Sampler:
+    """Samples (clock, power) every 50 ms on a thread while active."""
+
+    def __init__(self, sample):
+        self.sample, self.out = sample, []
+
+    def __enter__(self):
+        import threading
+
+        self.stop = threading.Event()
+
+        def run():
+            while not self.stop.is_set():
+                self.out.append(self.sample())
+                time.sleep(0.05)
+
+        self.th = threading.Thread(target=run, daemon=True)
+        self.th.start()
+        return self
+
+    def __exit__(self, *exc):
+        self.stop.set()
+        self.th.join()
+
+    def medians(self):
+        import statistics
+
+        tail = self.out[len(self.out) // 3:]  # past the ramp
+        med = lambda i: statistics.median(v[i] for v in tail if isinstance(v[i], (int, float))) if tail else None
+        return med(0), med(1)
 
 
 def bench_energy(args):
@@ -536,12 +573,15 @@ def bench_energy(args):
     op_n/op_n, A U[0,1), B FP16(U[0,1)); for tf32, fp16 and the library SGEMM,
     time 16 calls, then repeat the call for about --seconds and read the
     card's energy counter around that window.  Prints the reference's columns
-    (throughput, average W, J per GEMM, call count) plus GFLOP/J."""
+    (throughput, average W, J per GEMM, call count) plus GFLOP/J and the
+    median GFX clock and socket power sampled during the window; the FP16
+    GEMM of the platform library (torch.mm, hipBLASLt) is timed beside them as
+    a reference for what the card sustains under its power cap."""
     import torch
 
     import shgemm_amd as sg
 
-    joules = _energy_counter()
+    joules, sample = _energy_counter()
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     h = sg.create()
@@ -552,13 +592,15 @@ def bench_energy(args):
     A = torch.rand(top * top, device=dev, generator=g)
     B = torch.rand(top * top, device=dev, generator=g).half()
     B32 = B.float()
+    A16 = A.half()
     C = torch.empty(top * top, device=dev)
     window = args.seconds
     for log_n in range(10, 15):
         m = n = k = 1 << log_n
         calls = [("tf32", lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.tf32)),
                  ("fp16", lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16)),
-                 ("sgemm_fp32", lambda: sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, C, m))]
+                 ("sgemm_fp32", lambda: sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, C, m)),
+                 ("hipblaslt_hgemm_fp16", lambda: torch.mm(A16[:m * k].view(m, k), B[:k * n].view(k, n)))]
         for name, fn in calls:
             fn()
             torch.cuda.synchronize()
@@ -568,17 +610,20 @@ def bench_energy(args):
             torch.cuda.synchronize()
             per_call = (time.perf_counter() - t0) / 16
             count = max(1, int(window / per_call))
-            e0, t0 = joules(), time.perf_counter()
-            for _ in range(count):
-                fn()
-            torch.cuda.synchronize()
-            e1, t1 = joules(), time.perf_counter()
+            with _Sampler(sample) as smp:
+                e0, t0 = joules(), time.perf_counter()
+                for _ in range(count):
+                    fn()
+                torch.cuda.synchronize()
+                e1, t1 = joules(), time.perf_counter()
+            clk, pw = smp.medians()
             flop = 2.0 * m * n * k
             print(json.dumps({"compute_type": name, "m": m, "n": n, "k": k, "op_a": "N", "op_b": "N",
                               "throughput_tflops": round(flop / per_call / 1e12, 2),
                               "avg_power_w": round((e1 - e0) / (t1 - t0), 1),
                               "energy_per_gemm_j": (e1 - e0) / count, "test_count": count,
-                              "gflop_per_j": round(flop * count / (e1 - e0) / 1e9, 1)}), flush=True)
+                              "gflop_per_j": round(flop * count / (e1 - e0) / 1e9, 1),
+                              "median_gfxclk_mhz": clk, "median_socket_power_w": pw}), flush=True)
     sg.destroy(h)
 
 

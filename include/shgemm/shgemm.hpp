@@ -141,11 +141,15 @@ void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 //   (prescaled fp16, op_n/op_n, m and n multiples of 256, k of 32, aligned
 //   operands; otherwise it runs as 5), 7 large tile with the three-stage
 //   direct-to-LDS B pipeline (prescaled fp16, op_b = op_n, aligned B, k a
-//   multiple of 32; otherwise it runs as 1).
+//   multiple of 32; otherwise it runs as 1), 8 the 256x256 tile on a
+//   pre-split A: one pass writes A's FP16 hi / residual planes (4 bytes per
+//   element of A, in a handle buffer grown on first use) and the 256x256
+//   tile runs without a split (prescaled fp16, op_b = op_n, aligned B, k a
+//   multiple of 64; otherwise it runs as 5).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.
-enum { plan_variant_p2 = 5, plan_variant_p2r = 6, plan_variant_bd = 7, num_plan_variants = 8 };
+enum { plan_variant_p2 = 5, plan_variant_p2r = 6, plan_variant_bd = 7, plan_variant_ps = 8, num_plan_variants = 9 };
 enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
 struct plan_t {
   int variant = 1;

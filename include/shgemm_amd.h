@@ -82,12 +82,15 @@ int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on);
  * Plan variants: 0 small tile (P0), 1 large tile (P1), 2 128x128 split-K,
  * 3 large-tile split-K, 4 small-tile split-K, 5 256x256 tile of four
  * 128x128 wavefronts, 6 the same tile with a 3-slot direct-to-LDS ring for
- * A and B, 7 large tile with the three-stage direct-to-LDS B pipeline
- * (5-7 run as the nearest applicable variant where they do not apply).
+ * A and B, 7 large tile with the three-stage direct-to-LDS B pipeline,
+ * 8 the 256x256 tile on a pre-split A (A split once per call into a handle
+ * buffer of 4 bytes per element; prescaled fp16, op_b = N, k % 64 == 0)
+ * (5-8 run as the nearest applicable variant where they do not apply).
  * Plan sources (query): 0 cost model, 1 plan table, 2 forced, 3 fixed level. */
 #define SHGEMM_AMD_PLAN_P2 5
 #define SHGEMM_AMD_PLAN_P2R 6
 #define SHGEMM_AMD_PLAN_BD 7
+#define SHGEMM_AMD_PLAN_PS 8
 /* Force variant/slices for every later call on h (variant < 0 clears). */
 int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
 /* Kernel timing (no counterpart in the reference; for profiling): HIP events

@@ -367,4 +367,94 @@ hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int ns
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Pre-split of A for the pre-split plan: the prescaled FP16 hi / residual lo
+// of every element (the same values the GEMM kernels compute per tile),
+// written once per call in the GEMM's tile order.  A is cut into 256-row
+// blocks and 32-deep k-tiles; tile (bm, kt) is 32 KiB at
+// out + (bm * (k / 32) + kt) * 2048 slots of 16 bytes: 1024 hi slots then
+// 1024 lo slots, slot lds_slot_rm<4>(r, c) holding row r's k-values
+// 8c..8c+7 (the LDS image of the tile), rows past m zero.  A GEMM workgroup
+// then streams one contiguous 32 KiB block per k-tile.
+// Caller: k % 32 == 0.  One workgroup of 256 threads per tile.
+// ---------------------------------------------------------------------------
+constexpr int kPackRows = 256, kPackK = 32, kPackSlots = kPackRows * kPackK / 8;
+
+template <bool KCONT>
+__global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
+                                                            int vec, const int* __restrict__ row_exp,
+                                                            uint4* __restrict__ out) {
+  __shared__ float t[KCONT ? 1 : kPackK][kPackRows + 1];
+  const int mb = (m + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
+  const int bm = (int)(blockIdx.x % mb), kt = (int)(blockIdx.x / mb);
+  const int r0 = bm * kPackRows, k0 = kt * kPackK, tid = threadIdx.x;
+  if constexpr (!KCONT) {
+    // m-contiguous A: lane group g = tid / 64 takes columns g, g + 4, ...;
+    // each lane 4 consecutive rows (one 16-byte load where aligned and
+    // inside m), so a wavefront instruction reads one whole 1 KiB column
+    const int rr = 4 * (tid & 63), g = tid >> 6;
+    const bool full = vec && (r0 + rr + 4 <= m);
+#pragma unroll
+    for (int j = 0; j < kPackK / 4; ++j) {
+      const int col = g + 4 * j;
+      const float* src = a + (int64_t)(r0 + rr) + (int64_t)(k0 + col) * lda;
+      float x[4];
+      if (full) {
+        const float4 f = *reinterpret_cast<const float4*>(src);
+        x[0] = f.x; x[1] = f.y; x[2] = f.z; x[3] = f.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = r0 + rr + e < m ? src[e] : 0.0f;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) t[col][rr + e] = x[e];
+    }
+    __syncthreads();
+  }
+  uint4* tile = out + ((int64_t)bm * kt_n + kt) * (2 * kPackSlots);
+#pragma unroll
+  for (int q = 0; q < kPackSlots / 256; ++q) {
+    const int s = tid + 256 * q;
+    const int r = s / 4, c = (s % 4) ^ ((r / 4) & 3);  // inverse of lds_slot_rm<4>
+    float v[8];
+    if constexpr (KCONT) {
+      const bool in = r0 + r < m;
+      const float* src = a + (int64_t)(r0 + r) * lda + k0 + 8 * c;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = in ? src[e] : 0.0f;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = t[8 * c + e][r];
+    }
+    const float sc = r0 + r < m ? __builtin_ldexpf(1.0f, -row_exp[r0 + r]) : 1.0f;
+    uint4 h, l;
+    split2_f16_scaled(v[0], v[1], sc, h.x, l.x);
+    split2_f16_scaled(v[2], v[3], sc, h.y, l.y);
+    split2_f16_scaled(v[4], v[5], sc, h.z, l.z);
+    split2_f16_scaled(v[6], v[7], sc, h.w, l.w);
+    tile[s] = h;
+    tile[kPackSlots + s] = l;
+  }
+}
+
+size_t presplit_bytes(int m, int k) {
+  return (size_t)((m + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * 4;
+}
+
+hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, const int* row_exp, uint4* out,
+                             hipStream_t s) {
+  if (m <= 0 || k <= 0) return hipSuccess;
+  if (k % kPackK != 0) return hipErrorInvalidValue;
+  const int64_t blocks = (int64_t)((m + kPackRows - 1) / kPackRows) * (k / kPackK);
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  const int vec = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (lda % 4 == 0);
+  if (op_a == OP_T)
+    hipLaunchKernelGGL(presplit_pack_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp,
+                       out);
+  else
+    hipLaunchKernelGGL(presplit_pack_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp,
+                       out);
+  return hipGetLastError();
+}
+
 }  // namespace shgemm_amd

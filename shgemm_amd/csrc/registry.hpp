@@ -45,6 +45,8 @@ struct KernelTable {
   KernelEntry sgemm[2][2][2];
   // deep-prefetch prescaled fp16 kernel (op_b = n, full k-tiles): [op_a][op_c]
   KernelEntry shgemm_bd[2][2];
+  // pre-split prescaled fp16 kernel (A split once per call, op_b = n): [op_c]
+  KernelEntry shgemm_ps[2];
   // persistent grouped kernels: [op_a][op_b][op_c][mode][tile class]
   KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,
@@ -69,6 +71,7 @@ void register_shgemm_fp16s_tt(KernelTable&);
 void register_sgemm(KernelTable&);
 void register_tune(KernelTable&);
 void register_shgemm_bd(KernelTable&);
+void register_shgemm_ps(KernelTable&);
 void register_all(KernelTable&);
 
 // One problem of a group as the device sees it.  p.ws is the problem's own
@@ -125,6 +128,12 @@ hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_
 hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
                                 int* row_exp, uint32_t* partials, size_t partial_bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+// Prescaled FP16 hi / residual lo of A in the pre-split plan's tile order
+// (256-row blocks x 32-deep k-tiles, 32 KiB per tile); k % 32 == 0.  The
+// output needs presplit_bytes(m, k) bytes.
+size_t presplit_bytes(int m, int k);
+hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, const int* row_exp, uint4* out,
+                             hipStream_t s);
 hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s);
 hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,
                                  hipStream_t s);

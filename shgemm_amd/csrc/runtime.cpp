@@ -78,6 +78,7 @@ struct Plan {
   plan_source source = plan_cost_model;
   const shgemm_amd::KernelEntry* entry = nullptr;
   const char* note = "";
+  bool presplit = false;  // A split once per call (plan variant 8)
 };
 
 // Largest useful split: every slice at least 8 k-tiles, all partials inside
@@ -121,6 +122,18 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
+// The pre-split kernel: prescaled fp16, op_b = N with 16-byte-aligned B and
+// ldb % 8 == 0, and an even number of whole k-tiles.
+const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t k,
+                                        bool vec_b, int64_t ldb) {
+  const auto& e = st->table.shgemm_ps[oc];
+  (void)ldb;
+  if (e.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b && k >= 2 * e.bk &&
+      k % (2 * e.bk) == 0)
+    return &e;
+  return nullptr;
+}
+
 // The ring kernel runs whole 256 x 256 tiles and whole k-tiles (at least
 // two), with vector-aligned operands whose tile offsets fit 32 bits.
 bool ring_applies(const shgemm_amd::KernelEntry& e, int64_t m, int64_t n, int64_t k, bool vec_a, bool vec_b,
@@ -129,7 +142,7 @@ bool ring_applies(const shgemm_amd::KernelEntry& e, int64_t m, int64_t n, int64_
          (int64_t)e.bk * lda * 4 < (int64_t(1) << 32) && (int64_t)e.bn * ldb * 2 < (int64_t(1) << 32);
 }
 
-// A plan variant 0..7 with a slice count -> a launchable plan.  Split
+// A plan variant 0..8 with a slice count -> a launchable plan.  Split
 // variants get 2..max_slices slices; when k or the workspace allows no split
 // they run as the unsplit variant of the same tile.  Variants that do not
 // apply to the arguments run as the nearest one that does.
@@ -141,7 +154,16 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
   if (requested == shgemm_amd::V_P2R &&
       !ring_applies(tab[shgemm_amd::V_P2R], m, n, k, vec_a, vec_b, lda, ldb))
     requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
-  if (requested == plan_variant_bd) {
+  if (requested == plan_variant_ps) {
+    pl.entry = ps_entry(st, mode, ob, oc, k, vec_b, ldb);
+    if (pl.entry != nullptr) {
+      pl.variant = shgemm_amd::V_P2;
+      pl.presplit = true;
+      pl.note = " pre-split";
+    } else {
+      pl.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
+    }
+  } else if (requested == plan_variant_bd) {
     pl.variant = shgemm_amd::V_P1;
     pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
     if (pl.entry != nullptr) pl.note = " deep-prefetch";
@@ -297,6 +319,8 @@ void create(shgemmHandle_t& handle) {
   for (auto& a : st->table.shgemm_bd)
     for (auto& e : a)
       if (e.fn) set_lds_limit(e);
+  for (auto& e : st->table.shgemm_ps)
+    if (e.fn) set_lds_limit(e);
   const char* fp = std::getenv("MTK_SHGEMM_PLAN");
   if (fp != nullptr) {
     int v = -1, sl = 1;
@@ -324,6 +348,7 @@ void destroy(shgemmHandle_t& handle) {
   if (st->ws) (void)hipFree(st->ws);
   if (st->row_max) (void)hipFree(st->row_max);
   if (st->row_exp) (void)hipFree(st->row_exp);
+  if (st->ps_buf) (void)hipFree(st->ps_buf);
   release_timing(st);
   delete st;
   handle.state = nullptr;
@@ -380,7 +405,8 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
   const Plan pl = choose_plan(handle, st, st->table.shgemm[oa][ob][oc][mode], mode, oa, ob, oc, (int64_t)m, (int64_t)n,
                               (int64_t)k, true, true, lda, ldb);
   plan_t out;
-  out.variant = pl.entry != nullptr && pl.requested == plan_variant_bd ? plan_variant_bd : pl.variant;
+  out.variant = pl.presplit ? plan_variant_ps
+                : (pl.entry != nullptr && pl.requested == plan_variant_bd ? plan_variant_bd : pl.variant);
   out.slices = pl.slices;
   out.source = pl.source;
   return out;
@@ -450,8 +476,38 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   }
 
   const auto& tab = st->table.shgemm[oa][ob][oc][mode];
-  const Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
-                                p.vec_b != 0, p.lda, p.ldb);
+  Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
+                          p.vec_b != 0, p.lda, p.ldb);
+  if (plan.presplit) {
+    // A's packed hi and lo tiles, 4 bytes per element (m rounded up to 256),
+    // in a handle buffer grown on demand (growth waits for earlier work on
+    // the stream); if the device cannot hold them the call runs the
+    // register-split 256 x 256 tile.
+    const size_t bytes = shgemm_amd::presplit_bytes(p.m, p.k);
+    if (st->ps_bytes < bytes) {
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (st->ps_buf) (void)hipFree(st->ps_buf);
+      st->ps_buf = nullptr;
+      st->ps_bytes = 0;
+      if (hipMalloc(&st->ps_buf, bytes) == hipSuccess) {
+        st->ps_bytes = bytes;
+      } else {
+        (void)hipGetLastError();
+        st->ps_buf = nullptr;
+      }
+    }
+    if (st->ps_buf == nullptr) {
+      plan.presplit = false;
+      plan.entry = nullptr;
+      plan.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
+      plan.note = " (pre-split buffer unavailable)";
+    } else {
+      check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp,
+                                              reinterpret_cast<uint4*>(st->ps_buf), stream),
+                "pre-split pass");
+      p.a = reinterpret_cast<const float*>(st->ps_buf);  // the kernel reads the packed tiles
+    }
+  }
   const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
@@ -603,5 +659,6 @@ void register_all(KernelTable& t) {
   register_sgemm(t);
   register_tune(t);
   register_shgemm_bd(t);
+  register_shgemm_ps(t);
 }
 }  // namespace shgemm_amd

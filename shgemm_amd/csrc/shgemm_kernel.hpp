@@ -277,6 +277,19 @@ struct TileF16 {
     }
   }
 
+  // KCONT, full k-tile, rows past rlim clamped to the last row (their values
+  // only reach output rows or columns that are never stored).
+  __device__ __forceinline__ void load_rows_clamped(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
+                                                    int rlim, int tid) {
+    static_assert(KCONT, "k-contiguous rows");
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, c;
+      item(tid + NT * p, r, c);
+      vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)min(r0 + r, rlim - 1) * ld + (k0 + 8 * c));
+    }
+  }
+
   // 8 halves of row r + h (h = 0: low halves, 1: high halves) of a !KCONT item
   __device__ __forceinline__ u32x4 gather_row(int p, int h) const {
     uint32_t w[4];
@@ -995,6 +1008,156 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ring(const Ge
   int tm, tn;
   tile_coords(p, blockIdx.x, tm, tn);
   shgemm_tile_ring<OPC, CFG>(p, tm * CFG::BM, tn * CFG::BN, lds);
+}
+
+// A tile of the pre-split operand: SLOTS 16-byte slots already in LDS-image
+// order, copied by NT threads (slot tid + NT * p).
+template <int SLOTS, int NT>
+struct PackedTile {
+  static constexpr int PER = SLOTS / NT;
+  static_assert(SLOTS % NT == 0, "whole slots per thread");
+  u32x4 v[PER];
+  __device__ __forceinline__ void load(const uint4* __restrict__ src, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) v[p] = *reinterpret_cast<const u32x4*>(&src[tid + NT * p]);
+  }
+  __device__ __forceinline__ void store(uint4* __restrict__ dst, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) *reinterpret_cast<u32x4*>(&dst[tid + NT * p]) = v[p];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Pre-split kernel of the prescaled fp16 mode (256 x 256 x 32 tile, four
+// wavefronts of 128 x 128).  A arrives already split and packed
+// (launch_presplit_a: per 256-row block and k-tile, 32 KiB holding the hi
+// then lo LDS images; p.a points at it), so a k-tile is one contiguous A
+// block plus the B tile: 16-byte loads, two k-tiles ahead in two register
+// sets, stored to LDS unchanged -- no VALU split.  Per k-tile (stage cur holds
+// tile kt, one register set holds tile kt+1, the other receives tile kt+2):
+//   loads of tile kt+2 || reads of step 1 || MFMAs of step 0 || stores of
+//   tile kt+1 into stage cur^1;  barrier;  reads of step 0 of tile kt+1 ||
+//   MFMAs of step 1.
+// Each C element accumulates in the order of the register-split P2 tile, so
+// the results are bit-identical to it.  Rows past m are zero in the packed
+// A, rows past n are clamped by the B loader (never stored); the host
+// guarantees k % 64 == 0 (an even number of k-tiles) and B 16-byte aligned
+// with ldb % 8 == 0.
+// ---------------------------------------------------------------------------
+template <int OPC, class CFG>
+__device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __restrict__ lds) {
+  constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM;
+  constexpr bool LANE_M = (OPC == OP_N);
+  static_assert(KC == 4, "two k-steps per k-tile");
+  constexpr int A_CH = BM * KC, B_CH = BN * KC;
+  constexpr int STAGE = 2 * A_CH + B_CH;
+  using Frag = FragF16<FM, FN>;
+  constexpr int MF = FM * FN * 2;  // MFMAs per k-step
+  constexpr int NR = 2 * FM + FN;  // fragment reads per k-step
+  using TA = PackedTile<A_CH, NT>;
+  using TB = TileF16<true, BN, BK, NT>;
+  constexpr int NG = 2 * TA::PER + TB::PER;  // global loads per k-tile
+  constexpr int NW = NG;                     // LDS stores per k-tile
+
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int lrow = lane & 31, lh = lane >> 5;
+  const int ktiles = p.k / BK;
+  // this block row's packed k-tiles: 2 * A_CH slots each (hi, then lo)
+  const uint4* __restrict__ apack = reinterpret_cast<const uint4*>(p.a) + (int64_t)tm * ktiles * (2 * A_CH);
+  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+
+  f32x16 acc[1][FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[0][i][j][e] = 0.0f;
+
+  TA th[2], tl[2];
+  TB tb[2];
+  auto gload = [&](auto set_c, int kt) {
+    constexpr int set = decltype(set_c)::value;
+    const uint4* at = apack + (int64_t)kt * (2 * A_CH);
+    th[set].load(at, tid);
+    tl[set].load(at + A_CH, tid);
+    tb[set].load_rows_clamped(B, p.ldb, n0, kt * BK, p.n, tid);
+  };
+  auto sstore = [&](auto set_c, int stage) {
+    constexpr int set = decltype(set_c)::value;
+    uint4* st = lds + stage * STAGE;
+    th[set].store(st, tid);
+    tl[set].store(st + A_CH, tid);
+    tb[set].store_raw(st + 2 * A_CH, tid);
+  };
+  Frag f0, f1;
+  auto fload = [&](Frag& f, int stage, int step) {
+    const uint4* st = lds + stage * STAGE;
+    frag_load<FM, FN, KC, BM, WM, WN>(f, st, st + 2 * A_CH, step, wm, wn, lrow, lh);
+  };
+  // SETL: the register set that receives tile kt+2 (it held tile kt); the
+  // other set holds tile kt+1.  MORE: tile kt+2 exists; NEXT: tile kt+1 does.
+  auto ktile = [&](int kt, auto setl_c, auto more_c, auto next_c) {
+    constexpr int SETL = decltype(setl_c)::value;
+    constexpr bool MORE = decltype(more_c)::value, NEXT = decltype(next_c)::value;
+    using SN = std::integral_constant<int, 1 - SETL>;
+    const int cur = kt & 1;
+    if constexpr (MORE) gload(setl_c, kt + 2);
+    fload(f1, cur, 1);
+    frag_mma<LANE_M>(acc, f0);
+    if constexpr (NEXT) sstore(SN{}, cur ^ 1);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+#pragma unroll
+    for (int q = NR; q < MF; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+    if constexpr (NEXT) fload(f0, cur ^ 1, 0);
+    frag_mma<LANE_M>(acc, f1);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, MF - NR, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  static_assert(NW <= MF - NR, "every LDS store has an MFMA to follow");
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using T = std::true_type;
+  using F = std::false_type;
+  gload(S0{}, 0);
+  gload(S1{}, 1);  // ktiles >= 2
+  sstore(S0{}, 0);
+  __syncthreads();
+  fload(f0, 0, 0);
+  int kt = 0;
+  for (; kt + 3 < ktiles; kt += 2) {
+    ktile(kt, S0{}, T{}, T{});
+    ktile(kt + 1, S1{}, T{}, T{});
+  }
+  ktile(kt, S0{}, F{}, T{});  // ktiles is even
+  ktile(kt + 1, S1{}, F{}, F{});
+  store_ctile<MODE_FP16S, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
+}
+
+template <int OPC, class CFG>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const GemmParams p) {
+  extern __shared__ uint4 lds[];
+  shgemm_tile_ps<OPC, CFG>(p, lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -111,7 +111,7 @@ def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
         B = torch.randn(k * n, device="cuda").half()
         C = torch.empty(m * n, device="cuda")
         seen, best = set(), None
-        candidates = [(0, 1), (1, 1), (5, 1), (6, 1), (7, 1)] + [(v, s) for v in (2, 3, 4)
+        candidates = [(0, 1), (1, 1), (5, 1), (6, 1), (7, 1), (8, 1)] + [(v, s) for v in (2, 3, 4)
                                                                   for s in (2, 4, 8, 16, 32, 64, 128, 256)]
         for v, s in candidates:
             sg.force_plan(h, v, s)
@@ -266,7 +266,8 @@ def main(argv=None) -> int:
         f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}, {props.multi_processor_count} CUs)",
         f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
         f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
-        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage);"
+        "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage,"
+        " 8 P2 on pre-split A);"
         " time_us per call",
         "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large); time_us per problem in a full group",
     ]

@@ -132,6 +132,13 @@ void sgemm_fp32(
 // A * 2^-e(row) and the row of C is scaled back exactly.
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 
+// Device sharing (no counterpart in the reference): on a handle whose calls
+// run beside other kernels (several streams serving requests at once), the
+// plans whose 256x256 workgroups take a whole CU (variants 5, 6, 8) run as
+// the large 128x256 tile instead, which leaves room on each CU for the
+// concurrent work.  Off by default (the plan table is measured alone).
+void set_shared_device(shgemmHandle_t& handle, const bool on);
+
 // ---- Kernel plans --------------------------------------------------------
 // A plan is a kernel variant plus a k-slice count.  Variants:
 //   0 small tile (level P0), 1 large tile (P1), 2 128x128 split-K,

@@ -74,6 +74,10 @@ int shgemm_amd_sgemm_fp32(shgemm_amd_handle* h, int op_a, int op_b,
 /* No counterpart in the reference: per-row power-of-two prescaling of the
  * FP32 operand for the fp16 compute type (widens its exponent range). */
 int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on);
+/* Calls on h share the device with concurrent kernels: plans with
+ * whole-CU 256x256 workgroups run as the 128x256 tile (no counterpart in
+ * the reference; for servers running several streams at once). */
+int shgemm_amd_set_shared_device(shgemm_amd_handle* h, int on);
 
 /* ---- Kernel plans ---------------------------------------------------------
  * Replace the reference's static per-architecture tile tables and its

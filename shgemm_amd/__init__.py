@@ -78,6 +78,7 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_hsgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
     L.shgemm_amd_sgemm_fp32.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32]
     L.shgemm_amd_set_exponent_scaling.argtypes = [vp, i32]
+    L.shgemm_amd_set_shared_device.argtypes = [vp, i32]
     L.shgemm_amd_enable_kernel_timing.argtypes = [vp, i32]
     L.shgemm_amd_kernel_timing.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_float)]
     L.shgemm_amd_force_plan.argtypes = [vp, i32, i32]
@@ -189,6 +190,12 @@ def set_debug_mode(handle: shgemmHandle_t, on: int) -> None:
 
 def set_exponent_scaling(handle: shgemmHandle_t, on: bool) -> None:
     _check(load_library().shgemm_amd_set_exponent_scaling(handle.ptr, 1 if on else 0))
+
+
+def set_shared_device(handle: shgemmHandle_t, on: bool) -> None:
+    """Calls on `handle` run beside concurrent kernels: plans with whole-CU
+    256x256 workgroups run as the 128x256 tile instead."""
+    _check(load_library().shgemm_amd_set_shared_device(handle.ptr, 1 if on else 0))
 
 
 def shgemm(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, compute_type, op_c=op_n) -> kernel_level:

@@ -157,6 +157,13 @@ int shgemm_amd_set_exponent_scaling(shgemm_amd_handle* h, int on) {
   });
 }
 
+int shgemm_amd_set_shared_device(shgemm_amd_handle* h, int on) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::set_shared_device(h->h, on != 0);
+  });
+}
+
 int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices) {
   return guarded([&] {
     need(h);

@@ -151,6 +151,10 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
                   bool vec_a, bool vec_b, int64_t lda, int64_t ldb) {
   Plan pl;
   pl.requested = requested;
+  // a handle sharing the device runs the whole-CU 256 x 256 plans as P1
+  if (st->shared_device &&
+      (requested == shgemm_amd::V_P2 || requested == shgemm_amd::V_P2R || requested == plan_variant_ps))
+    requested = shgemm_amd::V_P1;
   if (requested == shgemm_amd::V_P2R &&
       !ring_applies(tab[shgemm_amd::V_P2R], m, n, k, vec_a, vec_b, lda, ldb))
     requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
@@ -226,6 +230,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   for (int v = 0; v < shgemm_amd::kNumVariants; ++v) {
     const auto& e = tab[v];
     if (e.fn == nullptr || v == shgemm_amd::V_P2R) continue;  // the ring variant is chosen by measurement only
+    if (v == shgemm_amd::V_P2 && st->shared_device) continue;
     const bool split = shgemm_amd::variant_is_split(v);
     const int64_t smax = split ? max_slices(e, m, n, k, st->ws_bytes) : 1;
     for (int64_t s = split ? 2 : 1; s <= smax; s *= 2) {
@@ -368,6 +373,11 @@ void set_cuda_stream(shgemmHandle_t& handle, hipStream_t const cuda_stream) { ha
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on) {
   if (handle.state == nullptr) throw error("handle not created");
   handle.state->exp_scaling = on;
+}
+
+void set_shared_device(shgemmHandle_t& handle, const bool on) {
+  if (handle.state == nullptr) throw error("handle not created");
+  handle.state->shared_device = on;
 }
 
 void force_plan(shgemmHandle_t& handle, const int variant, const int slices) {

@@ -26,6 +26,7 @@ struct handle_state {
   int* row_exp = nullptr;
   size_t row_cap = 0;
   bool exp_scaling = true;
+  bool shared_device = false;  // set_shared_device: no whole-CU 256x256 plans
   int tune_p1 = -1;         // MTK_SHGEMM_TUNE_P1: tile-sweep candidate for fp16 nn
   bool use_bd = false;      // MTK_SHGEMM_BD=1: deep-prefetch kernel where the cost model picks P1
   int forced_variant = -1;  // force_plan / MTK_SHGEMM_PLAN=<variant>,<slices>

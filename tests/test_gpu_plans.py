@@ -214,3 +214,19 @@ def test_presplit_plan_falls_back_where_it_does_not_apply(h):
     assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[0] == 1   # two-accumulator mode
     sg.set_exponent_scaling(h, True)
     sg.force_plan(h, -1, 1)
+
+
+def test_shared_device_avoids_whole_cu_plans(h):
+    import shgemm_amd as sg
+
+    sg.force_plan(h, 8, 1)
+    assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] == 8
+    sg.set_shared_device(h, True)
+    for v in (5, 6, 8):
+        sg.force_plan(h, v, 1)
+        assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] == 1
+    sg.force_plan(h, -1, 1)
+    assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] not in (5, 6, 8)
+    sg.load_plans(h, None)  # cost model
+    assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] != 5
+    sg.set_shared_device(h, False)

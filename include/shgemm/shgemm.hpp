@@ -149,10 +149,11 @@ void set_shared_device(shgemmHandle_t& handle, const bool on);
 //   operands; otherwise it runs as 5), 7 large tile with the three-stage
 //   direct-to-LDS B pipeline (prescaled fp16, op_b = op_n, aligned B, k a
 //   multiple of 32; otherwise it runs as 1), 8 the 256x256 tile on a
-//   pre-split A: one pass writes A's FP16 hi / residual planes (4 bytes per
-//   element of A, in a handle buffer grown on first use) and the 256x256
-//   tile runs without a split (prescaled fp16, op_b = op_n, aligned B, k a
-//   multiple of 64; otherwise it runs as 5).
+//   pre-split A: one pass writes A's FP16 hi / residual in the tile order
+//   of the kernel and one packs B the same way (4 bytes per element of A
+//   and 2 per element of B, in a handle buffer grown on first use), then
+//   the 256x256 tile runs without a split (prescaled fp16, k a multiple of
+//   64; otherwise it runs as 5).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.

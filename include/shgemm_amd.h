@@ -87,8 +87,9 @@ int shgemm_amd_set_shared_device(shgemm_amd_handle* h, int on);
  * 3 large-tile split-K, 4 small-tile split-K, 5 256x256 tile of four
  * 128x128 wavefronts, 6 the same tile with a 3-slot direct-to-LDS ring for
  * A and B, 7 large tile with the three-stage direct-to-LDS B pipeline,
- * 8 the 256x256 tile on a pre-split A (A split once per call into a handle
- * buffer of 4 bytes per element; prescaled fp16, op_b = N, k % 64 == 0)
+ * 8 the 256x256 tile on a pre-split A (A split and B packed once per call
+ * into a handle buffer of 4 bytes per element of A and 2 per element of B;
+ * prescaled fp16, k % 64 == 0)
  * (5-8 run as the nearest applicable variant where they do not apply).
  * Plan sources (query): 0 cost model, 1 plan table, 2 forced, 3 fixed level. */
 #define SHGEMM_AMD_PLAN_P2 5

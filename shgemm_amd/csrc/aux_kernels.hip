@@ -437,6 +437,80 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
   }
 }
 
+// B of the pre-split plan in the same tile order: per 256-column block of n
+// and 32-deep k-tile, 16 KiB holding the tile's LDS image (slot
+// lds_slot_rm<4>(j, c) = column j's k-values 8c..8c+7, FP16 bits unchanged),
+// columns past n zero.  B is k x n with op_b applied: op_n stores element
+// (kk, j) at b[kk + j * ldb] (k contiguous), op_t at b[j + kk * ldb].
+template <bool KCONT>
+__global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict__ b, int64_t ldb, int n, int k, int vec,
+                                                     uint4* __restrict__ out) {
+  __shared__ uint16_t t[KCONT ? 1 : kPackK][kPackRows + 8];
+  const int nb = (n + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
+  const int bn = (int)(blockIdx.x % nb), kt = (int)(blockIdx.x / nb);
+  const int j0 = bn * kPackRows, k0 = kt * kPackK, tid = threadIdx.x;
+  if constexpr (!KCONT) {
+    // n-contiguous B: each thread loads 8 consecutive columns of one k-row
+#pragma unroll
+    for (int q = 0; q < kPackSlots / 256; ++q) {
+      const int i = tid + 256 * q;
+      const int kk = i / (kPackRows / 8), jj = 8 * (i % (kPackRows / 8));
+      const uint16_t* src = b + (int64_t)(j0 + jj) + (int64_t)(k0 + kk) * ldb;
+      if (vec && j0 + jj + 8 <= n) {
+        const uint4 w = *reinterpret_cast<const uint4*>(src);
+        *reinterpret_cast<uint4*>(&t[kk][jj]) = w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) t[kk][jj + e] = j0 + jj + e < n ? src[e] : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+  }
+  uint4* tile = out + ((int64_t)bn * kt_n + kt) * kPackSlots;
+#pragma unroll
+  for (int q = 0; q < kPackSlots / 256; ++q) {
+    const int s = tid + 256 * q;
+    const int j = s / 4, c = (s % 4) ^ ((j / 4) & 3);  // inverse of lds_slot_rm<4>
+    uint4 w;
+    if constexpr (KCONT) {
+      const uint16_t* src = b + (int64_t)(j0 + j) * ldb + k0 + 8 * c;
+      if (j0 + j >= n) {
+        w = uint4{0u, 0u, 0u, 0u};
+      } else if (vec) {
+        w = *reinterpret_cast<const uint4*>(src);
+      } else {
+        uint32_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = (uint32_t)src[2 * e] | ((uint32_t)src[2 * e + 1] << 16);
+        w = uint4{h[0], h[1], h[2], h[3]};
+      }
+    } else {
+      uint32_t h[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) h[e] = (uint32_t)t[8 * c + 2 * e][j] | ((uint32_t)t[8 * c + 2 * e + 1][j] << 16);
+      w = uint4{h[0], h[1], h[2], h[3]};
+    }
+    tile[s] = w;
+  }
+}
+
+size_t pack_b_bytes(int n, int k) {
+  return (size_t)((n + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * 2;
+}
+
+hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, uint4* out, hipStream_t s) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  if (k % kPackK != 0) return hipErrorInvalidValue;
+  const int64_t blocks = (int64_t)((n + kPackRows - 1) / kPackRows) * (k / kPackK);
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  const int vec = (reinterpret_cast<uintptr_t>(b) % 16 == 0) && (ldb % 8 == 0);
+  if (op_b == OP_N)
+    hipLaunchKernelGGL(pack_b_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, vec, out);
+  else
+    hipLaunchKernelGGL(pack_b_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, vec, out);
+  return hipGetLastError();
+}
+
 size_t presplit_bytes(int m, int k) {
   return (size_t)((m + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * 4;
 }

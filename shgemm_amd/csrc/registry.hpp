@@ -134,6 +134,10 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 size_t presplit_bytes(int m, int k);
 hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, const int* row_exp, uint4* out,
                              hipStream_t s);
+// B (FP16, k x n after op_b) in the same tile order: 256-column blocks x
+// 32-deep k-tiles, 16 KiB per tile; k % 32 == 0; pack_b_bytes(n, k) bytes.
+size_t pack_b_bytes(int n, int k);
+hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, uint4* out, hipStream_t s);
 hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s);
 hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,
                                  hipStream_t s);

@@ -122,15 +122,11 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
-// The pre-split kernel: prescaled fp16, op_b = N with 16-byte-aligned B and
-// ldb % 8 == 0, and an even number of whole k-tiles.
-const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t k,
-                                        bool vec_b, int64_t ldb) {
+// The pre-split kernel: prescaled fp16 and an even number of whole k-tiles
+// (A and B are packed by the plan's passes, whatever their layouts).
+const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int oc, int64_t k) {
   const auto& e = st->table.shgemm_ps[oc];
-  (void)ldb;
-  if (e.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b && k >= 2 * e.bk &&
-      k % (2 * e.bk) == 0)
-    return &e;
+  if (e.fn != nullptr && mode == shgemm_amd::MODE_FP16S && k >= 2 * e.bk && k % (2 * e.bk) == 0) return &e;
   return nullptr;
 }
 
@@ -159,7 +155,7 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
       !ring_applies(tab[shgemm_amd::V_P2R], m, n, k, vec_a, vec_b, lda, ldb))
     requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
   if (requested == plan_variant_ps) {
-    pl.entry = ps_entry(st, mode, ob, oc, k, vec_b, ldb);
+    pl.entry = ps_entry(st, mode, oc, k);
     if (pl.entry != nullptr) {
       pl.variant = shgemm_amd::V_P2;
       pl.presplit = true;
@@ -489,11 +485,13 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
                           p.vec_b != 0, p.lda, p.ldb);
   if (plan.presplit) {
-    // A's packed hi and lo tiles, 4 bytes per element (m rounded up to 256),
-    // in a handle buffer grown on demand (growth waits for earlier work on
-    // the stream); if the device cannot hold them the call runs the
+    // A's packed hi and lo tiles (4 bytes per element, m rounded up to 256)
+    // and B's packed tiles (2 bytes per element, n rounded up to 256), in a
+    // handle buffer grown on demand (growth waits for earlier work on the
+    // stream); if the device cannot hold them the call runs the
     // register-split 256 x 256 tile.
-    const size_t bytes = shgemm_amd::presplit_bytes(p.m, p.k);
+    const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k);
+    const size_t bytes = a_bytes + shgemm_amd::pack_b_bytes(p.n, p.k);
     if (st->ps_bytes < bytes) {
       check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
       if (st->ps_buf) (void)hipFree(st->ps_buf);
@@ -512,10 +510,15 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
       plan.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
       plan.note = " (pre-split buffer unavailable)";
     } else {
-      check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp,
-                                              reinterpret_cast<uint4*>(st->ps_buf), stream),
+      uint4* apack = reinterpret_cast<uint4*>(st->ps_buf);
+      uint4* bpack = reinterpret_cast<uint4*>(reinterpret_cast<char*>(st->ps_buf) + a_bytes);
+      check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp, apack, stream),
                 "pre-split pass");
-      p.a = reinterpret_cast<const float*>(st->ps_buf);  // the kernel reads the packed tiles
+      check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k, bpack,
+                                          stream),
+                "B pack pass");
+      p.a = reinterpret_cast<const float*>(apack);  // the kernel reads the packed tiles
+      p.b = bpack;
     }
   }
   const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];

@@ -33,7 +33,7 @@ struct handle_state {
   int forced_slices = 1;
   std::vector<shgemm_amd::PlanRow> plans;  // measured plan table
   std::string plans_path;
-  // pre-split plan: A packed as FP16 hi / lo tiles (grown on demand)
+  // pre-split plan: A packed as FP16 hi / lo tiles, then B packed (grown on demand)
   uint16_t* ps_buf = nullptr;
   size_t ps_bytes = 0;
   // enable_kernel_timing: event pairs recorded around the main GEMM kernel

@@ -277,19 +277,6 @@ struct TileF16 {
     }
   }
 
-  // KCONT, full k-tile, rows past rlim clamped to the last row (their values
-  // only reach output rows or columns that are never stored).
-  __device__ __forceinline__ void load_rows_clamped(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
-                                                    int rlim, int tid) {
-    static_assert(KCONT, "k-contiguous rows");
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int r, c;
-      item(tid + NT * p, r, c);
-      vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)min(r0 + r, rlim - 1) * ld + (k0 + 8 * c));
-    }
-  }
-
   // 8 halves of row r + h (h = 0: low halves, 1: high halves) of a !KCONT item
   __device__ __forceinline__ u32x4 gather_row(int p, int h) const {
     uint32_t w[4];
@@ -1031,18 +1018,19 @@ struct PackedTile {
 // Pre-split kernel of the prescaled fp16 mode (256 x 256 x 32 tile, four
 // wavefronts of 128 x 128).  A arrives already split and packed
 // (launch_presplit_a: per 256-row block and k-tile, 32 KiB holding the hi
-// then lo LDS images; p.a points at it), so a k-tile is one contiguous A
-// block plus the B tile: 16-byte loads, two k-tiles ahead in two register
-// sets, stored to LDS unchanged -- no VALU split.  Per k-tile (stage cur holds
+// then lo LDS images; p.a points at it) and B packed the same way
+// (launch_pack_b: 16 KiB per 256-column block and k-tile; p.b points at
+// it), so a k-tile is one contiguous A block and one contiguous B block:
+// 16-byte loads, two k-tiles ahead in two register sets, stored to LDS
+// unchanged -- no VALU split.  Per k-tile (stage cur holds
 // tile kt, one register set holds tile kt+1, the other receives tile kt+2):
 //   loads of tile kt+2 || reads of step 1 || MFMAs of step 0 || stores of
 //   tile kt+1 into stage cur^1;  barrier;  reads of step 0 of tile kt+1 ||
 //   MFMAs of step 1.
 // Each C element accumulates in the order of the register-split P2 tile, so
-// the results are bit-identical to it.  Rows past m are zero in the packed
-// A, rows past n are clamped by the B loader (never stored); the host
-// guarantees k % 64 == 0 (an even number of k-tiles) and B 16-byte aligned
-// with ldb % 8 == 0.
+// the results are bit-identical to it.  Rows past m and columns past n are
+// zero in the packed operands (never stored); the host guarantees k % 64 ==
+// 0 (an even number of k-tiles).
 // ---------------------------------------------------------------------------
 template <int OPC, class CFG>
 __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __restrict__ lds) {
@@ -1056,7 +1044,7 @@ __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __res
   constexpr int MF = FM * FN * 2;  // MFMAs per k-step
   constexpr int NR = 2 * FM + FN;  // fragment reads per k-step
   using TA = PackedTile<A_CH, NT>;
-  using TB = TileF16<true, BN, BK, NT>;
+  using TB = PackedTile<B_CH, NT>;
   constexpr int NG = 2 * TA::PER + TB::PER;  // global loads per k-tile
   constexpr int NW = NG;                     // LDS stores per k-tile
 
@@ -1069,7 +1057,8 @@ __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __res
   const int ktiles = p.k / BK;
   // this block row's packed k-tiles: 2 * A_CH slots each (hi, then lo)
   const uint4* __restrict__ apack = reinterpret_cast<const uint4*>(p.a) + (int64_t)tm * ktiles * (2 * A_CH);
-  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+  // this block column's packed B k-tiles: B_CH slots each
+  const uint4* __restrict__ bpack = static_cast<const uint4*>(p.b) + (int64_t)tn * ktiles * B_CH;
 
   f32x16 acc[1][FM][FN];
 #pragma unroll
@@ -1086,14 +1075,14 @@ __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __res
     const uint4* at = apack + (int64_t)kt * (2 * A_CH);
     th[set].load(at, tid);
     tl[set].load(at + A_CH, tid);
-    tb[set].load_rows_clamped(B, p.ldb, n0, kt * BK, p.n, tid);
+    tb[set].load(bpack + (int64_t)kt * B_CH, tid);
   };
   auto sstore = [&](auto set_c, int stage) {
     constexpr int set = decltype(set_c)::value;
     uint4* st = lds + stage * STAGE;
     th[set].store(st, tid);
     tl[set].store(st + A_CH, tid);
-    tb[set].store_raw(st + 2 * A_CH, tid);
+    tb[set].store(st + 2 * A_CH, tid);
   };
   Frag f0, f1;
   auto fload = [&](Frag& f, int stage, int step) {

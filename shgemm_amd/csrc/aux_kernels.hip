@@ -93,9 +93,21 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
         }
       }
     } else {
-      for (int kk = k0 + kl; kk < k1; kk += klanes)
-        for (int e = 0; e < 4; ++e)
-          if (r4 + e < m) mx[e] = umax(mx[e], abs_bits(a[r4 + e + (int64_t)kk * lda]));
+      // unaligned rows or lda % 4 != 0: 4-byte loads, still 8 columns in flight
+      constexpr int U = 8;
+      for (int kk = k0 + kl; kk < k1; kk += U * klanes) {
+        float v[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kx = kk + u * klanes;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[u][e] = (kx < k1 && r4 + e < m) ? a[r4 + e + (int64_t)kx * lda] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) mx[e] = umax(mx[e], abs_bits(v[u][e]));
+      }
     }
   }
   // combine the k-lanes of each row-quad in LDS, then one store (or atomic) per row
@@ -140,6 +152,7 @@ __device__ __forceinline__ void rowmax_kcontig_block(const float* __restrict__ a
     }
     for (int kk = max(v1, v0) + threadIdx.x; kk < k1; kk += 256) mx = umax(mx, abs_bits(row[kk]));
   } else {
+#pragma unroll 8
     for (int kk = k0 + threadIdx.x; kk < k1; kk += 256) mx = umax(mx, abs_bits(row[kk]));
   }
   __shared__ uint32_t red[256];

@@ -248,6 +248,28 @@ struct TileF16 {
     }
   }
 
+  // Full tile, any ld and base alignment: 2-byte loads, no bounds checks.
+  __device__ __forceinline__ void load_fast_u16(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
+                                                int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, c;
+      item(tid + NT * p, r, c);
+      if constexpr (KCONT) {
+        const uint16_t* ptr = base + (int64_t)(r0 + r) * ld + (k0 + 8 * c);
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = (uint32_t)ptr[2 * q] | ((uint32_t)ptr[2 * q + 1] << 16);
+        vk[p] = u32x4{w[0], w[1], w[2], w[3]};
+      } else {
+        const uint16_t* ptr = base + (r0 + r) + (int64_t)(k0 + 8 * c) * ld;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          vr[p][e] = (uint32_t)ptr[(int64_t)e * ld] | ((uint32_t)ptr[(int64_t)e * ld + 1] << 16);
+      }
+    }
+  }
+
   __device__ __forceinline__ void load_safe(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
                                             int rlim, int klim, int tid) {
 #pragma unroll
@@ -631,7 +653,9 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const float* __restrict__ A = p.a;
   const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
-  const bool a_full = p.vec_a && (m0 + BM <= p.m);
+  // unchecked loads: op_n A uses 4-byte loads (any lda); the 16-byte forms
+  // need vec_a / vec_b
+  const bool a_full = (OPA == OP_N || p.vec_a) && (m0 + BM <= p.m);
   const bool b_full = p.vec_b && (n0 + BN <= p.n);
 
   if constexpr (SCALED) {
@@ -693,17 +717,23 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     // wait for the A loads before issuing the B loads) nor splits the loads
     // across blocks.
     const int kfull = (kend - kbeg) / BK;
-    if (a_full && b_full) {
+    auto fast_loop = [&](auto bload) {
       for (; kt + 1 < kfull; ++kt) {
         const int cur = kt & 1;
         const int k1 = kbeg + (kt + 1) * BK;
         ta[0].load_fast(A, p.lda, m0, k1, tid);
-        tb[0].load_fast(B, p.ldb, n0, k1, tid);
+        bload(k1);
         __builtin_amdgcn_sched_barrier(0);  // loads go out before the MFMA work
         compute(lds + cur * STAGE);
         sstore(S0{}, cur ^ 1);
         __syncthreads();
       }
+    };
+    if (a_full && b_full) {
+      fast_loop([&](int k1) { tb[0].load_fast(B, p.ldb, n0, k1, tid); });
+    } else if (a_full && n0 + BN <= p.n) {
+      // B rows whole but ldb or the base not 16-byte aligned
+      fast_loop([&](int k1) { tb[0].load_fast_u16(B, p.ldb, n0, k1, tid); });
     }
     for (; kt < ktiles; ++kt) {
       const int cur = kt & 1;

@@ -71,12 +71,17 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
                                                      int kchunk, int rq, int vec, uint32_t* __restrict__ dst,
                                                      int bx, int by) {
   const int q = threadIdx.x % rq, kl = threadIdx.x / rq, klanes = 256 / rq;
+  // rows of this thread: a quad r4..r4+3 for 16-byte loads (vec), otherwise
+  // rows q, q + rq, q + 2rq, q + 3rq of the block, so that each 4-byte load
+  // instruction reads consecutive rows
   const int r4 = (bx * rq + q) * 4;
+  const int rb = bx * rq * 4 + q;
+  auto row = [&](int e) { return vec ? r4 + e : rb + e * rq; };
   uint32_t mx[4] = {0u, 0u, 0u, 0u};
-  if (r4 < m && kl < klanes) {
+  if (kl < klanes) {
     const int k0 = by * kchunk, k1 = min(k, k0 + kchunk);
+    constexpr int U = 8;
     if (vec && r4 + 4 <= m) {
-      constexpr int U = 8;
       for (int kk = k0 + kl; kk < k1; kk += U * klanes) {
         float4 v[U];
 #pragma unroll
@@ -93,15 +98,14 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
         }
       }
     } else {
-      // unaligned rows or lda % 4 != 0: 4-byte loads, still 8 columns in flight
-      constexpr int U = 8;
+      // 4-byte loads, 8 columns in flight
       for (int kk = k0 + kl; kk < k1; kk += U * klanes) {
         float v[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int kx = kk + u * klanes;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) v[u][e] = (kx < k1 && r4 + e < m) ? a[r4 + e + (int64_t)kx * lda] : 0.0f;
+          for (int e = 0; e < 4; ++e) v[u][e] = (kx < k1 && row(e) < m) ? a[row(e) + (int64_t)kx * lda] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -110,21 +114,24 @@ __device__ __forceinline__ void rowmax_rcontig_block(const float* __restrict__ a
       }
     }
   }
-  // combine the k-lanes of each row-quad in LDS, then one store (or atomic) per row
+  // combine the k-lanes of each thread's rows in LDS, then one store (or
+  // atomic) per row
   __shared__ uint32_t red[4][256];
   for (int e = 0; e < 4; ++e) red[e][threadIdx.x] = mx[e];
   __syncthreads();
-  if (kl == 0 && r4 < m) {
+  if (kl == 0) {
     for (int e = 0; e < 4; ++e) {
       uint32_t v = 0;
       for (int j = 0; j < klanes; ++j) v = umax(v, red[e][q + j * rq]);
-      if (r4 + e < m) {
-        if constexpr (ATOMIC) atomicMax(&dst[r4 + e], v);
-        else dst[r4 + e] = v;
+      const int r = row(e);
+      if (r < m) {
+        if constexpr (ATOMIC) atomicMax(&dst[r], v);
+        else dst[r] = v;
       }
     }
   }
 }
+
 
 __global__ void __launch_bounds__(256) rowmax_rcontig_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
                                                               int kchunk, int rq, int vec,

@@ -27,6 +27,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_FP16_DENSE = 2516.6   # TFLOP/s, MI355X FP16 dense MFMA
 PEAK_FP32_MATRIX = 157.3   # TFLOP/s, MI355X FP32 matrix
 PEAK_HBM = 8.0             # TB/s
+P99_BOUND_MS = 10.0        # stated sketch-server latency bound (p99, ms) up to the reported rate
 
 
 def parse():
@@ -308,6 +309,10 @@ def bench_serve(args):
         "sustained_req_s": round(head["sustained_req_s"], 1),
         "offered_req_s": head["offered_rate"],
         "p50_ms": round(head["p50_ms"], 3),
+        # the stated bound: p99 within P99_BOUND_MS at every offered rate the
+        # server sustains (>= 95% of the offered rate)
+        "p99_bound_ms": P99_BOUND_MS,
+        "p99_within_bound": all(r["p99_ms"] <= P99_BOUND_MS for r in ok) and bool(ok),
         "per_rate": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
         "saturation": {"requests": sat["requests"], "req_s": round(sat["sustained_req_s"], 1),
                        "tflops": round(sat["flops_per_s"] / 1e12, 2)},

@@ -273,7 +273,9 @@ static RowmaxGeometry rowmax_partial_geometry(int op_a, int m, int k, int64_t pa
     while (g.rq < 256 && g.rq < quads) g.rq *= 2;
     g.bx = (quads + g.rq - 1) / g.rq;
     rb = g.bx;
-    kb_min_chunk = 256 / g.rq;  // one load per k-lane at least
+    // one load per k-lane at least, and 8192 elements per workgroup so
+    // small matrices need few partials (one reduction level)
+    kb_min_chunk = std::max(256 / g.rq, 8192 / (4 * g.rq));
   } else {
     g.rq = 0;
     g.bx = m;

@@ -180,18 +180,6 @@ __global__ void __launch_bounds__(256) rowmax_kcontig_kernel(const float* __rest
   rowmax_kcontig_block<false>(a, lda, k, kchunk, vec, partial + (int64_t)blockIdx.y * m, blockIdx.x, blockIdx.y);
 }
 
-// row_exp = ilogb(max |A[r,:]|) - 14 from the |a| bit pattern of the maximum
-// (0 for an all-zero or non-finite row), clamped so 2^-e stays a finite FP32.
-__device__ __forceinline__ int exponent_of_max(uint32_t b) {
-  int e = 0;
-  if (b != 0 && b < 0x7f800000u) {  // finite, non-zero
-    const int biased = (int)(b >> 23);
-    const int lg = biased != 0 ? biased - 127 : (31 - __builtin_clz(b)) - 149;  // ilogb(|a|max)
-    e = lg - 14;
-    e = e < -126 ? -126 : e;
-  }
-  return e;
-}
 
 __global__ void __launch_bounds__(256) rowexp_kernel(const uint32_t* __restrict__ row_max, int64_t m,
                                                       int* __restrict__ row_exp) {

@@ -113,6 +113,19 @@ __device__ __forceinline__ void split2_f16_scaled(float x0, float x1, float s, u
       : "v"(x0), "v"(x1), "v"(s));
 }
 
+// row_exp = ilogb(max |A[r,:]|) - 14 from the |a| bit pattern of the maximum
+// (0 for an all-zero or non-finite row), clamped so 2^-e stays a finite FP32.
+__device__ __forceinline__ int exponent_of_max(uint32_t b) {
+  int e = 0;
+  if (b != 0 && b < 0x7f800000u) {  // finite, non-zero
+    const int biased = (int)(b >> 23);
+    const int lg = biased != 0 ? biased - 127 : (31 - __builtin_clz(b)) - 149;  // ilogb(|a|max)
+    e = lg - 14;
+    e = e < -126 ? -126 : e;
+  }
+  return e;
+}
+
 // FP32 -> three BF16 parts a0 + a1 + a2 (each residual exact in FP32).
 __device__ __forceinline__ void split_bf16x3(float a, __bf16& a0, __bf16& a1, __bf16& a2) {
   a0 = (__bf16)a;

@@ -131,7 +131,7 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
   struct Planned {
     size_t idx;
     int cls, used;
-    int64_t kslice, units, row_off, ws_off;  // ws_off in floats, -1: no partials
+    int64_t kslice, units, row_off, ws_off;  // row_off: -1 no row exponents; ws_off in floats, -1: no partials
   };
   std::vector<Planned> work;        // problems with MFMA work
   std::vector<size_t> scale_only;   // k == 0 or alpha == 0: C = beta * C
@@ -163,12 +163,14 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
     const int64_t kslice = s > 1 ? ((k + s - 1) / s + e.bk - 1) / e.bk * e.bk : k;
     const int used = (int)((k + kslice - 1) / kslice);
     const int64_t tiles = (int64_t)((q.m + e.bm - 1) / e.bm) * (int64_t)((q.n + e.bn - 1) / e.bn);
-    Planned pl{i, c, used, used > 1 ? kslice : k, tiles * used, rows, -1};
+    // prescaled kernels that read row exponents (the others keep their own)
+    const bool rexp = scaled && e.needs_row_exp;
+    Planned pl{i, c, used, used > 1 ? kslice : k, tiles * used, rexp ? rows : -1, -1};
     if (used > 1) {
       pl.ws_off = ws_floats;
       ws_floats += (int64_t)used * (int64_t)q.m * (int64_t)q.n;
     }
-    if (scaled) rows += (int64_t)q.m;
+    if (rexp) rows += (int64_t)q.m;
     g->cls[i] = c;
     g->slices[i] = used;
     work.push_back(pl);
@@ -190,9 +192,11 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
   // ---- row-max segments -----------------------------------------------------
   std::vector<shgemm_amd::RowmaxSeg> rm;
   std::vector<int> rm_begin;
+  std::vector<int64_t> rm_row_off;
   int64_t rm_blocks = 0;
   if (scaled) {
     for (const Planned& pl : work) {
+      if (pl.row_off < 0) continue;
       const group_problem& q = problems[pl.idx];
       const shgemm_amd::RowmaxGeometry geo = shgemm_amd::rowmax_geometry(oa, (int)q.m, (int)q.k);
       shgemm_amd::RowmaxSeg sg{};
@@ -209,6 +213,7 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
       rm_blocks += (int64_t)geo.bx * geo.by;
       if (rm_blocks > INT_MAX) throw error("group row-exponent pass too large");
       rm.push_back(sg);
+      rm_row_off.push_back(pl.row_off);
     }
   }
 
@@ -287,7 +292,7 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
       p.c = q.C;
       p.ldc = (int64_t)q.ldc;
       p.ws = pl.used > 1 ? ws + pl.ws_off : nullptr;
-      p.row_exp = scaled ? g->row_exp + pl.row_off : nullptr;
+      p.row_exp = pl.row_off >= 0 ? g->row_exp + pl.row_off : nullptr;
       p.k_slice = (int)pl.kslice;
       p.tiles_m = (int)((q.m + e.bm - 1) / e.bm);
       p.tiles_n = (int)((q.n + e.bn - 1) / e.bn);
@@ -322,7 +327,7 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
     std::memcpy(host.data() + off_rmb, rm_begin.data(), rm_begin.size() * sizeof(int));
     // each problem's row maxima start at its row offset
     auto* hrm = reinterpret_cast<shgemm_amd::RowmaxSeg*>(host.data() + off_rm);
-    for (size_t j = 0; j < work.size(); ++j) hrm[j].row_max = g->row_max + work[j].row_off;
+    for (size_t j = 0; j < rm.size(); ++j) hrm[j].row_max = g->row_max + rm_row_off[j];
     g->rm = reinterpret_cast<shgemm_amd::RowmaxSeg*>(base + off_rm);
     g->rm_begin = reinterpret_cast<int*>(base + off_rmb);
     g->rm_n = (int)rm.size();

@@ -12,6 +12,7 @@ static void put_bd(KernelTable& t) {
   e.bk = CfgBD::BK;
   e.mode = MODE_FP16S;
   e.threads = CfgBD::NT;
+  e.needs_row_exp = true;
 }
 void register_shgemm_bd(KernelTable& t) {
   put_bd<OP_N, OP_N>(t);

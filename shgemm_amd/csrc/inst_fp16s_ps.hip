@@ -14,6 +14,7 @@ static void put_ps(KernelTable& t) {
   e.bk = CfgPS::BK;
   e.mode = MODE_FP16S;
   e.threads = CfgPS::NT;
+  e.needs_row_exp = true;
   e.wgs_per_cu = 1;
 }
 void register_shgemm_ps(KernelTable& t) {

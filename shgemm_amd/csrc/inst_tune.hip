@@ -14,6 +14,7 @@ static void put_tune(KernelTable& t) {
   e.bk = CFG::BK;
   e.mode = MODE;
   e.threads = CFG::NT;
+  e.needs_row_exp = MODE == MODE_FP16S && !CFG::template running<MODE>();
 }
 // Deep-prefetch kernel (B direct to a 3-stage LDS ring, A in two register sets).
 template <class CFG>
@@ -28,6 +29,7 @@ static void put_tune_bd(KernelTable& t) {
   e.bk = CFG::BK;
   e.mode = MODE_FP16S;
   e.threads = CFG::NT;
+  e.needs_row_exp = true;
 }
 // Ring-pipeline kernel (raw A and B straight to LDS rings; interior tiles).
 template <class CFG>
@@ -42,6 +44,7 @@ static void put_tune_ring(KernelTable& t) {
   e.bk = CFG::BK;
   e.mode = MODE_FP16S;
   e.threads = CFG::NT;
+  e.needs_row_exp = true;
 }
 void register_tune(KernelTable& t) {
   t.ntune = 0;

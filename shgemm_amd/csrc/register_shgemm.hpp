@@ -19,6 +19,7 @@ inline void put_shgemm(KernelTable& t, int variant) {
   e.mode = MODE;
   e.threads = CFG::NT;
   e.wgs_per_cu = (4 * CFG::OCC) / CFG::WAVES > 0 ? (4 * CFG::OCC) / CFG::WAVES : 1;
+  e.needs_row_exp = MODE == MODE_FP16S && !CFG::template running<MODE>();
 }
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG>
@@ -32,6 +33,7 @@ inline void put_grouped(KernelTable& t, int cls) {
   e.bk = CFG::BK;
   e.mode = MODE;
   e.threads = CFG::NT;
+  e.needs_row_exp = MODE == MODE_FP16S && !CFG::template running<MODE>();
 }
 
 template <int OPC, class CFG>
@@ -46,6 +48,7 @@ inline void put_ring(KernelTable& t) {
   e.mode = MODE_FP16S;
   e.threads = CFG::NT;
   e.wgs_per_cu = 1;
+  e.needs_row_exp = true;
 }
 
 template <int OPA, int OPB, int OPC, int MODE>

@@ -17,6 +17,7 @@ struct KernelEntry {
   int mode = -1;
   int threads = 256;
   int wgs_per_cu = 2;  // workgroups one CU holds at the configuration's register budget
+  bool needs_row_exp = false;  // prescaled fp16 kernel that reads exponents from the row pass
 };
 
 enum : int { LEVEL_P0 = 0, LEVEL_P1 = 1 };

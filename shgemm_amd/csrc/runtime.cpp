@@ -460,27 +460,6 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   p.vec_a = shgemm_amd::vec_a_ok(a_ptr, lda);
   p.vec_b = shgemm_amd::vec_b_ok(b_ptr, ldb, ob);
 
-  if (scaled) {
-    if (st->row_cap < m) {
-      // Grow the prescale scratch; earlier work on this handle may still use it.
-      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      if (st->row_max) (void)hipFree(st->row_max);
-      if (st->row_exp) (void)hipFree(st->row_exp);
-      st->row_max = nullptr;
-      st->row_exp = nullptr;
-      st->row_cap = 0;
-      check_hip(hipMalloc(&st->row_max, m * sizeof(uint32_t)), "hipMalloc(row scratch)");
-      check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
-      st->row_cap = m;
-    }
-    // the k-block partial maxima live in the split-K workspace: this call's
-    // GEMM uses it only after the exponent pass, in stream order
-    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp,
-                                               reinterpret_cast<uint32_t*>(st->ws), st->ws_bytes, stream),
-              "row exponent pass");
-    p.row_exp = st->row_exp;
-  }
-
   const auto& tab = st->table.shgemm[oa][ob][oc][mode];
   Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
                           p.vec_b != 0, p.lda, p.ldb);
@@ -509,19 +488,44 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
       plan.entry = nullptr;
       plan.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
       plan.note = " (pre-split buffer unavailable)";
-    } else {
-      uint4* apack = reinterpret_cast<uint4*>(st->ps_buf);
-      uint4* bpack = reinterpret_cast<uint4*>(reinterpret_cast<char*>(st->ps_buf) + a_bytes);
-      check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp, apack, stream),
-                "pre-split pass");
-      check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k, bpack,
-                                          stream),
-                "B pack pass");
-      p.a = reinterpret_cast<const float*>(apack);  // the kernel reads the packed tiles
-      p.b = bpack;
     }
   }
   const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];
+  // Prescaled fp16: the kernels with one k-tile of prefetch keep running row
+  // exponents themselves; the others read them from a pass over A.
+  if (scaled && e.needs_row_exp) {
+    if (st->row_cap < m) {
+      // Grow the prescale scratch; earlier work on this handle may still use it.
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (st->row_max) (void)hipFree(st->row_max);
+      if (st->row_exp) (void)hipFree(st->row_exp);
+      st->row_max = nullptr;
+      st->row_exp = nullptr;
+      st->row_cap = 0;
+      check_hip(hipMalloc(&st->row_max, m * sizeof(uint32_t)), "hipMalloc(row scratch)");
+      check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
+      st->row_cap = m;
+    }
+    // the k-block partial maxima live in the split-K workspace: this call's
+    // GEMM uses it only after the exponent pass, in stream order
+    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp,
+                                               reinterpret_cast<uint32_t*>(st->ws), st->ws_bytes, stream),
+              "row exponent pass");
+    p.row_exp = st->row_exp;
+  }
+
+  if (plan.presplit) {
+    const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k);
+    uint4* apack = reinterpret_cast<uint4*>(st->ps_buf);
+    uint4* bpack = reinterpret_cast<uint4*>(reinterpret_cast<char*>(st->ps_buf) + a_bytes);
+    check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp, apack, stream),
+              "pre-split pass");
+    check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k, bpack,
+                                        stream),
+              "B pack pass");
+    p.a = reinterpret_cast<const float*>(apack);  // the kernel reads the packed tiles
+    p.b = bpack;
+  }
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
   p.group_m = group_rows(p.tiles_m);

@@ -72,6 +72,17 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
     }
   }
 
+  // |a| bits of the largest of item p's 8 values
+  __device__ __forceinline__ uint32_t item_max(int p) const {
+    uint32_t mx = 0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint32_t b = __builtin_bit_cast(uint32_t, v[p][e]) & 0x7fffffffu;
+      mx = b > mx ? b : mx;
+    }
+    return mx;
+  }
+
   __device__ __forceinline__ void load_fast(const float* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -393,6 +404,20 @@ __device__ __forceinline__ constexpr int vmcnt_imm(int n) {
   return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
 }
 
+// Running row exponents of one output tile (LDS, after the two stages):
+// by k-tile parity the maxima |a| (bits, LDS atomic max) of the chunks that
+// exceed their row's exponent and the index of the last k-tile with such a
+// chunk; per row the current exponent and the accumulator factor
+// 2^(old - new) of the last rise.
+template <int BM>
+struct RunState {
+  uint32_t rmax[2][BM];
+  int erow[BM];
+  float fac[BM];
+  int flag[2];
+  int pad[14];
+};
+
 // ---------------------------------------------------------------------------
 // Tile configuration
 // ---------------------------------------------------------------------------
@@ -418,9 +443,16 @@ struct TileCfg {
   static constexpr int stage_chunks() {
     return (MODE == MODE_BF16 ? 3 : 2) * BM * KC + (MODE == MODE_BF16 ? 2 : 1) * BN * KC;
   }
+  // Prescaled fp16 with one k-tile of prefetch: the tile keeps a running
+  // exponent per row itself (RunState in LDS after the two stages) instead of
+  // reading exponents from a separate pass over A.
+  template <int MODE>
+  static constexpr bool running() {
+    return MODE == MODE_FP16S && PF == 1;
+  }
   template <int MODE>
   static constexpr size_t lds_bytes() {
-    return 2 * (size_t)stage_chunks<MODE>() * 16;
+    return 2 * (size_t)stage_chunks<MODE>() * 16 + (running<MODE>() ? sizeof(RunState<BM>) : 0);
   }
 };
 
@@ -452,8 +484,8 @@ struct ModeCfg {
 template <>
 struct ModeCfg<MODE_FP16S> {
   using P1 = TileCfg<128, 256, 32, 64, 64, 1, 4>;  // 8 wavefronts, 2 workgroups (4 wavefronts) per SIMD
-  using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
-  using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
+  using P0 = TileCfg<64, 64, 64, 32, 32, 1, 4>;
+  using SK = TileCfg<128, 128, 32, 64, 64, 1, 3>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 2, 1, true>;
   static constexpr bool kHasP2 = true;
 };
@@ -580,10 +612,11 @@ __device__ __forceinline__ void mma_ktile(f32x16 (&acc)[NACC][FM][FN], const uin
 // index gives the column, register e gives row (e/4)*8 + (lane/32)*4 + e%4.
 // TO_WS: the tile is one k-slice's partial; it goes unscaled by alpha/beta to
 // `part` (m*n floats in C's storage order, leading dimension m or n).
-template <int MODE, bool LANE_M, bool TO_WS, int NACC, int FM, int FN, int WM, int WN>
+template <int MODE, bool LANE_M, bool TO_WS, int NACC, int FM, int FN, int WM, int WN, bool ROWEXP = true>
 __device__ __forceinline__ void store_ctile(const f32x16 (&acc)[NACC][FM][FN], const GemmParams& p, int m0, int n0,
                                             int wm, int wn, int lrow, int lh, float* __restrict__ part) {
-  constexpr bool SCALED = (MODE == MODE_FP16S);
+  // ROWEXP = false: the accumulators already carry their rows' exponents
+  constexpr bool SCALED = (MODE == MODE_FP16S) && ROWEXP;
   constexpr float lo_w = 1.0f / 2048.0f;
   const float alpha = p.alpha, beta = p.beta;
 #pragma unroll
@@ -658,7 +691,42 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   const bool a_full = (OPA == OP_N || p.vec_a) && (m0 + BM <= p.m);
   const bool b_full = p.vec_b && (n0 + BN <= p.n);
 
-  if constexpr (SCALED) {
+  // RUN: running row exponents kept by the tile (no separate pass over A).
+  // Every thread keeps the current exponent of its items' rows (ecur, the
+  // same in every thread holding a chunk of the row) and splits each k-tile
+  // with it.  A chunk whose maximum needs a larger exponent merges that
+  // maximum into rmax[t&1] and records t in flag[t&1] before the barrier
+  // that ends the k-tile's stores; after it the workgroup reads the flag
+  // (uniform) and, only when it is set, gives each row the exponent
+  // max(ecur, that of rmax), splits the k-tile again from the registers into
+  // the same stage, records the factor 2^(old - new) and waits at one more
+  // barrier.  Before the k-tile's MFMAs the accumulators of a tile that rose
+  // are scaled by their rows' factors (exact powers of two).  Rows start at
+  // the floor -126 (all-zero rows keep it).  flag[t&1] is written again two
+  // k-tiles later, after every read of it; rmax[t&1] is cleared after the
+  // extra barrier, before the next barrier.
+  constexpr bool RUN = CFG::template running<MODE>();
+  constexpr int kFloor = -126;
+  using RS = RunState<BM>;
+  using MI = ItemMap<OPA == OP_T, BM, BK, NT>;
+  RS* rs = nullptr;
+  int ecur[MI::PER];
+  if constexpr (RUN) {
+    static_assert(PF == 1, "running exponents use one register set");
+    rs = reinterpret_cast<RS*>(lds + 2 * STAGE);
+    for (int r = tid; r < BM; r += NT) {
+      rs->rmax[0][r] = 0u;
+      rs->rmax[1][r] = 0u;
+      rs->erow[r] = kFloor;
+    }
+    if (tid < 2) rs->flag[tid] = -1;
+#pragma unroll
+    for (int q = 0; q < MI::PER; ++q) {
+      ecur[q] = kFloor;
+      ta[0].scale[q] = __builtin_ldexpf(1.0f, -kFloor);
+    }
+    __syncthreads();
+  } else if constexpr (SCALED) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) ta[i].init_scale(p.row_exp, m0, p.m, tid);
   }
@@ -695,6 +763,78 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       tb[set].store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
     }
   };
+  // RUN, before k-tile t (in register set 0) is stored: flag the chunks that
+  // need a larger exponent (par_c == t & 1, an int or an integral_constant)
+  auto run_check = [&](int t, auto par_c) {
+    const int par = static_cast<int>(par_c);
+#pragma unroll
+    for (int q = 0; q < MI::PER; ++q) {
+      const uint32_t b = ta[0].item_max(q);
+      const int et = b == 0u ? kFloor : exponent_of_max(b);
+      if (et > ecur[q]) {
+        int r, c;
+        MI::item(tid + NT * q, r, c);
+        atomicMax(&rs->rmax[par][r], b);
+        rs->flag[par] = t;
+      }
+    }
+  };
+  // RUN, after the barrier that ends k-tile t's stores into `stage`: whether
+  // some row's exponent rose at t (uniform); if so the new exponents, the
+  // factors and the k-tile split again (contains a barrier)
+  auto run_settle = [&](int t, auto par_c, int stage) -> bool {
+    const int par = static_cast<int>(par_c);
+    if (__builtin_amdgcn_readfirstlane(rs->flag[par]) != t) return false;
+#pragma unroll
+    for (int q = 0; q < MI::PER; ++q) {
+      int r, c;
+      MI::item(tid + NT * q, r, c);
+      const uint32_t b = rs->rmax[par][r];
+      const int et = b == 0u ? kFloor : exponent_of_max(b);
+      const int en = et > ecur[q] ? et : ecur[q];
+      if (c == 0) {
+        rs->fac[r] = __builtin_ldexpf(1.0f, ecur[q] - en);
+        rs->erow[r] = en;
+      }
+      ecur[q] = en;
+      ta[0].scale[q] = __builtin_ldexpf(1.0f, -en);
+    }
+    sstore(S0{}, stage);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MI::PER; ++q) {
+      int r, c;
+      MI::item(tid + NT * q, r, c);
+      if (c == 0) rs->rmax[par][r] = 0u;
+    }
+    return true;
+  };
+  // RUN: before the MFMAs of a k-tile at which some row's exponent rose,
+  // every accumulator times its row's factor 2^(old - new) (1 for most rows)
+  auto run_rescale = [&]() {
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      if constexpr (LANE_M) {
+        const float f = rs->fac[wm * WM + i * 32 + lrow];  // the lane's row in fragment row block i
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[0][i][j][e] *= f;
+      } else {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {  // registers 4g..4g+3 hold rows 8g + 4lh + 0..3
+          const float4 f = *reinterpret_cast<const float4*>(&rs->fac[wm * WM + i * 32 + g * 8 + lh * 4]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc[0][i][j][4 * g + 0] *= f.x;
+            acc[0][i][j][4 * g + 1] *= f.y;
+            acc[0][i][j][4 * g + 2] *= f.z;
+            acc[0][i][j][4 * g + 3] *= f.w;
+          }
+        }
+      }
+    }
+  };
   auto compute = [&](const uint4* __restrict__ st) {
     mma_ktile<MODE, LANE_M, NACC, FM, FN, KC, BM, BN, WM, WN>(acc, st, st + NA * A_CH, wm, wn, lrow, lh);
   };
@@ -705,28 +845,51 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   // before tile kt is multiplied and stored to LDS only after tile kt+1 is,
   // so each load has two k-tiles of MFMA work to land behind.
   if constexpr (PF == 1) {
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    bool rose = false;  // RUN: some row's exponent rose at the next k-tile to multiply
     if (ktiles > 0) {
       gload(S0{}, 0);
+      if constexpr (RUN) run_check(0, P0{});
       sstore(S0{}, 0);
     }
     __syncthreads();
+    if constexpr (RUN) {
+      if (ktiles > 0) run_settle(0, P0{}, 0);  // nothing accumulated yet: no rescale
+    }
     int kt = 0;
     // Interior tiles while the next k-tile is full: only the unchecked loads,
     // with no branch between the A and B loads, so the compiler neither
     // merges the checked paths' register state into the loop (which made it
     // wait for the A loads before issuing the B loads) nor splits the loads
-    // across blocks.
+    // across blocks.  Two k-tiles per trip, so each one's LDS stage (and
+    // running-exponent parity) is a constant: the LDS addresses are fixed
+    // per-lane offsets plus immediates, with no per-tile address arithmetic.
     const int kfull = (kend - kbeg) / BK;
+    auto fast_step = [&](auto cur_c, auto bload) {
+      constexpr int cur = decltype(cur_c)::value;  // == kt & 1
+      using NX = std::integral_constant<int, cur ^ 1>;
+      const int k1 = kbeg + (kt + 1) * BK;
+      ta[0].load_fast(A, p.lda, m0, k1, tid);
+      bload(k1);
+      __builtin_amdgcn_sched_barrier(0);  // loads go out before the MFMA work
+      if constexpr (RUN) {
+        if (rose) run_rescale();
+      }
+      compute(lds + cur * STAGE);
+      if constexpr (RUN) {
+        __builtin_amdgcn_sched_barrier(0);  // the exponent checks wait for the loads after the MFMAs
+        run_check(kt + 1, NX{});
+      }
+      sstore(S0{}, cur ^ 1);
+      __syncthreads();
+      if constexpr (RUN) rose = run_settle(kt + 1, NX{}, cur ^ 1);
+      ++kt;
+    };
     auto fast_loop = [&](auto bload) {
-      for (; kt + 1 < kfull; ++kt) {
-        const int cur = kt & 1;
-        const int k1 = kbeg + (kt + 1) * BK;
-        ta[0].load_fast(A, p.lda, m0, k1, tid);
-        bload(k1);
-        __builtin_amdgcn_sched_barrier(0);  // loads go out before the MFMA work
-        compute(lds + cur * STAGE);
-        sstore(S0{}, cur ^ 1);
-        __syncthreads();
+      for (; kt + 2 < kfull;) {  // kt is even here
+        fast_step(P0{}, bload);
+        fast_step(P1{}, bload);
       }
     };
     if (a_full && b_full) {
@@ -738,9 +901,18 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     for (; kt < ktiles; ++kt) {
       const int cur = kt & 1;
       if (kt + 1 < ktiles) gload(S0{}, kt + 1);
+      if constexpr (RUN) {
+        if (rose) run_rescale();
+      }
       compute(lds + cur * STAGE);
-      if (kt + 1 < ktiles) sstore(S0{}, cur ^ 1);
+      if (kt + 1 < ktiles) {
+        if constexpr (RUN) run_check(kt + 1, cur ^ 1);
+        sstore(S0{}, cur ^ 1);
+      }
       __syncthreads();
+      if constexpr (RUN) {
+        if (kt + 1 < ktiles) rose = run_settle(kt + 1, cur ^ 1, cur ^ 1);
+      }
     }
   } else {
     if (ktiles > 0) gload(S0{}, 0);
@@ -844,11 +1016,44 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     }
   }
 
+  if constexpr (RUN) {
+    // the rows' final exponents onto the accumulators (exact powers of two);
+    // erow last changed before a barrier
+    if (ktiles > 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        if constexpr (LANE_M) {
+          const int e = rs->erow[wm * WM + i * 32 + lrow];
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int x = 0; x < 16; ++x) acc[0][i][j][x] = __builtin_ldexpf(acc[0][i][j][x], e);
+        } else {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {  // registers 4g..4g+3 hold rows 8g + 4lh + 0..3
+            const int4 e = *reinterpret_cast<const int4*>(&rs->erow[wm * WM + i * 32 + g * 8 + lh * 4]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+              acc[0][i][j][4 * g + 0] = __builtin_ldexpf(acc[0][i][j][4 * g + 0], e.x);
+              acc[0][i][j][4 * g + 1] = __builtin_ldexpf(acc[0][i][j][4 * g + 1], e.y);
+              acc[0][i][j][4 * g + 2] = __builtin_ldexpf(acc[0][i][j][4 * g + 2], e.z);
+              acc[0][i][j][4 * g + 3] = __builtin_ldexpf(acc[0][i][j][4 * g + 3], e.w);
+            }
+          }
+        }
+      }
+    }
+    // the RunState is read above: no thread may reuse it (next grouped unit)
+    // before every thread has finished here
+    __syncthreads();
+  }
+  constexpr bool ROWEXP = !RUN;
   if constexpr (SPLIT == 2) {
-    if (part != nullptr) store_ctile<MODE, LANE_M, true, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
-    else store_ctile<MODE, LANE_M, false, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+    if (part != nullptr)
+      store_ctile<MODE, LANE_M, true, NACC, FM, FN, WM, WN, ROWEXP>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+    else store_ctile<MODE, LANE_M, false, NACC, FM, FN, WM, WN, ROWEXP>(acc, p, m0, n0, wm, wn, lrow, lh, part);
   } else {
-    store_ctile<MODE, LANE_M, SPLIT == 1, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, part);
+    store_ctile<MODE, LANE_M, SPLIT == 1, NACC, FM, FN, WM, WN, ROWEXP>(acc, p, m0, n0, wm, wn, lrow, lh, part);
   }
 }
 

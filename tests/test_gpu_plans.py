@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.gpu_util import dev_buffer, read_buffer, require_gpu
+from tests.gpu_util import dev_buffer, read_buffer, require_gpu, run_sgemm
 
 pytestmark = pytest.mark.gpu
 COST, TABLE, FORCED, FIXED = 0, 1, 2, 3
@@ -121,10 +121,14 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
     (plan 6, on whole tiles: the 512x768x256 case), the large tile (plan 1)
     and the three-stage pipeline (plan 7, where it applies) accumulate every C
     element over k in the same order, so unsplit results are bit-identical;
-    odd sizes exercise every edge path."""
+    odd sizes exercise every edge path.  With per-row prescaling the large
+    tile keeps running row exponents (its k-tiles are scaled by the maximum
+    so far, not by the row's final maximum): plans 5-7 still agree bitwise
+    and plan 1 is held to the FP64 bar (2x the library SGEMM's error)."""
     import shgemm_amd as sg
 
     op_a, op_b, op_c = ops
+    running = ct == 0 and scaling
     rng = np.random.default_rng(100 + 8 * ct + 4 * op_a + 2 * op_b + op_c)
     for m, n, k in [(300, 517, 333), (1000, 600, 704), (257, 255, 96), (512, 768, 256)]:
         lda, ldb, ldc = (m if op_a == 0 else k) + 1, (k if op_b == 0 else n) + 2, (m if op_c == 0 else n) + 3
@@ -141,8 +145,69 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
             outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())
         sg.force_plan(h, -1, 1)
         sg.set_exponent_scaling(h, True)
+        base = outs[1] if running else outs[0]
         for o in outs[1:]:
-            assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32)), (m, n, k)
+            assert np.array_equal(o.view(np.uint32), base.view(np.uint32)), (m, n, k)
+        if running:
+            B32 = B16.astype(np.float32)
+            ref = oracle.op_view(A.astype(np.float64), op_a, m, k, lda) @ oracle.op_view(
+                B32.astype(np.float64), op_b, k, n, ldb)
+            Cs = run_sgemm(h, op_a, op_b, op_c, m, n, k, 1.0, A, lda, B32, ldb, 0.0,
+                           np.zeros(c_t.numel(), np.float32), ldc)
+            err = oracle.rel_fro_error(oracle.c_matrix(outs[0], op_c, m, n, ldc), ref)
+            err_s = oracle.rel_fro_error(oracle.c_matrix(Cs, op_c, m, n, ldc), ref)
+            assert err <= 2.0 * err_s, (m, n, k, err, err_s)
+
+
+def _growing_rows(rng, m, k):
+    """A (m x k, column-major) whose rows change scale along k: row r has
+    magnitude 2^(s_r * t) in k-tile t (32 columns), s_r in {-3, 0, 3, 7};
+    every fifth row starts with two all-zero k-tiles and one row is zero."""
+    t = np.arange(k) // 32
+    slope = np.array([-3, 0, 3, 7])[np.arange(m) % 4]
+    expo = np.clip(slope[:, None] * t[None, :] - 20, -60, 60)
+    A = rng.uniform(-1.0, 1.0, (m, k)) * np.exp2(expo)
+    A[::5, :64] = 0.0
+    A[m // 2, :] = 0.0
+    return np.asfortranarray(A.astype(np.float32)).ravel(order="F")
+
+
+@pytest.mark.parametrize("op_a,op_c", [(0, 0), (1, 1), (0, 1)])
+@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (2, 3), (3, 2), (4, 4)])
+def test_running_exponents_follow_growing_rows(h, op_a, op_c, plan, slices):
+    """The kernels with running row exponents rescale their accumulators when
+    a row's maximum grows along k: rows growing, shrinking or starting with
+    zero k-tiles stay within 2x of the library SGEMM's FP64 error, unsplit
+    and split over k."""
+    import shgemm_amd as sg
+
+    m, n, k = 301, 200, 544
+    rng = np.random.default_rng(40 + 10 * plan + 2 * op_a + op_c)
+    A0 = _growing_rows(rng, m, k)                      # column-major m x k
+    A = A0 if op_a == 0 else np.ascontiguousarray(A0.reshape(k, m).T).ravel()  # op_t: stored k x m
+    lda = m if op_a == 0 else k
+    ldc = m if op_c == 0 else n
+    B16 = oracle.gen_f16_normal(rng, k * n)
+    B32 = B16.astype(np.float32)
+    a_t, a_p = dev_buffer(A)
+    b_t, b_p = dev_buffer(B16)
+    c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
+    sg.force_plan(h, plan, slices)
+    try:
+        sg.shgemm(h, op_a, 0, m, n, k, 1.0, a_p, lda, b_p, k, 0.0, c_p, ldc, sg.fp16, op_c)
+    finally:
+        sg.force_plan(h, -1, 1)
+    got = oracle.c_matrix(read_buffer(c_t, 0, m * n), op_c, m, n, ldc)
+    ref = oracle.op_view(A.astype(np.float64), op_a, m, k, lda) @ B32.astype(np.float64).reshape(n, k).T
+    Cs = run_sgemm(h, op_a, 0, op_c, m, n, k, 1.0, A, lda, B32, k, 0.0, np.zeros(m * n, np.float32), ldc)
+    # row-relative errors: the rows differ by up to 2^400 in scale
+    diff = np.abs(got - ref).max(axis=1)
+    diff_s = np.abs(oracle.c_matrix(Cs, op_c, m, n, ldc) - ref).max(axis=1)
+    scale = np.abs(ref).max(axis=1)
+    live = scale > 0
+    assert np.all(got[~live] == 0.0)
+    r, r_s = diff[live] / scale[live], diff_s[live] / scale[live]
+    assert np.all(r <= 2.0 * r_s + 2.0 ** -22), (plan, slices, float(r.max()), float(r_s.max()))
 
 
 def test_kernel_timing_counts_main_kernel_launches(h):

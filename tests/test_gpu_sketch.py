@@ -113,7 +113,9 @@ def test_server_results_same_alone_and_in_mix(torch_mod):
     rng = np.random.default_rng(21)
     mix_reqs = _requests(torch, pool, ld, 24, rng)
 
-    srv = SketchServer(hbm_budget=8e9, slots=1, lanes=GpuLanes(1))
+    # one request at a time on the same four-lane configuration (the lanes'
+    # handles choose kernels for a shared device, as in the mix below)
+    srv = SketchServer(hbm_budget=8e9, slots=4, lanes=GpuLanes(4))
     alone = []
     for r in alone_reqs:
         srv.submit(r)

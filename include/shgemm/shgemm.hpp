@@ -127,9 +127,12 @@ void sgemm_fp32(
 
 // Per-row power-of-two prescaling of the FP32 operand for the fp16 compute
 // type.  On by default (MTK_SHGEMM_EXP_SCALING=0 turns it off at create, which
-// gives the reference's fp16 mode with FP16's exponent range).  When on, one
-// extra pass over A computes each row's exponent; the split runs on
-// A * 2^-e(row) and the row of C is scaled back exactly.
+// gives the reference's fp16 mode with FP16's exponent range).  When on, the
+// split runs on A * 2^-e and the row of C is scaled back exactly: the
+// one-deep kernels (plans 0-4, grouped) keep a running exponent per row over
+// their k-range (rescaling their accumulator rows by exact powers of two when
+// it rises); the 256x256, ring, three-stage and pre-split kernels (plans 5-8)
+// take each row's whole-row exponent from one extra pass over A.
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 
 // Device sharing (no counterpart in the reference): on a handle whose calls
@@ -192,10 +195,10 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
 // take (problem, k-slice, tile) work units from a queue, longest first.  Each
 // problem gets a tile class (0 small, 1 128x128, 2 the large tile) and a
 // k-slice count from the plan table's grouped entries (or the arguments).
-// create_group validates every problem, allocates the group's descriptors,
-// row-exponent scratch and split-K partials, and captures the whole sequence
-// (row-exponent pass, one launch per tile class used, split-K reduction) in
-// one HIP graph; run_group replays it on `stream`.  Results are bit-identical
+// create_group validates every problem, allocates the group's descriptors
+// and split-K partials, and captures the whole sequence (one launch per tile
+// class used, split-K reduction; the tile classes keep running row
+// exponents) in one HIP graph; run_group replays it on `stream`.  Results are bit-identical
 // to running each problem alone with the same tile depth and slice count.
 struct group_problem {
   std::size_t m = 0, n = 0, k = 0;

@@ -11,7 +11,9 @@ Contents
   split_f16                  hi/lo FP16 split (reference src/wmmae_shgemm.hpp:29-31)
   split_bf16x3, split_bf16x2 BF16 parts of the tf32 compute type
   emulate_shgemm             FP64 evaluation of the split representation the
-                             GPU multiplies: alpha*(op(hi) + 2^-11 op(lo)) op(B) + beta*C0
+                             GPU multiplies: alpha*(op(hi) + 2^-11 op(lo)) op(B) + beta*C0,
+                             and the prescaled forms (whole-row exponents,
+                             running_row_exponents of the one-deep kernels)
   rel_fro_error              ||C - C64||_F / ||C64||_F (the error metric)
   philox4x32_10, philox_words, gaussian
                              the sketch generator (Philox4x32-10 + Box-Muller),
@@ -163,6 +165,43 @@ def split_value_f16_rowscaled(Am: np.ndarray) -> np.ndarray:
     return np.ldexp(v.reshape(Am.shape), e[:, None])
 
 
+def running_row_exponents(Am: np.ndarray, ktile: int, kslice: int | None = None) -> np.ndarray:
+    """Running row exponents of the one-deep kernels (the prescaled fp16 P0,
+    P1, split-K and grouped tiles): an m x T array, T = number of ktile-wide
+    k-tiles, whose entry (r, t) is the exponent k-tile t of row r is split
+    with: max(previous entry, ilogb(max |tile|) - 14), clamped below at -126,
+    an all-zero tile contributing -126 and a non-finite maximum 0 (as the
+    whole-row row_exponents does).  Each k-slice of a split-K product (kslice
+    columns, a multiple of ktile) starts again from -126."""
+    Am = np.asarray(Am, dtype=np.float32)
+    m, k = Am.shape
+    span = kslice if kslice else k
+    tiles = (k + ktile - 1) // ktile
+    out = np.empty((m, tiles), np.int64)
+    cur = np.full(m, -126, np.int64)
+    for t in range(tiles):
+        if (t * ktile) % span == 0:
+            cur[:] = -126
+        mx = np.abs(Am[:, t * ktile:(t + 1) * ktile]).max(axis=1).astype(np.float64)
+        _, ex = np.frexp(mx)                      # mx = f * 2^ex, f in [0.5, 1)
+        et = np.maximum(ex.astype(np.int64) - 1 - 14, -126)
+        et = np.where(mx == 0.0, -126, np.where(np.isfinite(mx), et, 0))
+        cur = np.maximum(cur, et)
+        out[:, t] = cur
+    return out
+
+
+def split_value_f16_running(Am: np.ndarray, ktile: int, kslice: int | None = None) -> np.ndarray:
+    """FP64 value of the running-exponent split (hi + lo) * 2^e(r, t) of an
+    m x k matrix (e from running_row_exponents)."""
+    Am = np.asarray(Am, dtype=np.float32)
+    e = np.repeat(running_row_exponents(Am, ktile, kslice), ktile, axis=1)[:, :Am.shape[1]]
+    xs = np.ldexp(Am, -e).astype(np.float32)
+    hi, lo = split_f16_same_scale(xs)
+    v = f16_bits_to_f32(hi).astype(np.float64) + f16_bits_to_f32(lo).astype(np.float64)
+    return np.ldexp(v.reshape(Am.shape), e)
+
+
 def split_bf16x3(a: np.ndarray):
     a = _check_f32(a.ravel(), "a")
     p = [np.empty(a.size, np.uint16) for _ in range(3)]
@@ -191,20 +230,25 @@ def split_value_f16(a: np.ndarray) -> np.ndarray:
     return f16_bits_to_f32(hi).astype(np.float64) + np.ldexp(f16_bits_to_f32(lo).astype(np.float64), -11)
 
 
-def emulate_shgemm(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, compute_type=0):
+def emulate_shgemm(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, compute_type=0,
+                   ktile=32, kslice=None):
     """FP64 product of the operands exactly as the GPU kernel represents them.
 
     compute_type 0 (fp16, prescaling off): A -> hi + 2^-11 lo (FP16 each).
     compute_type 1 (tf32): A -> a0 + a1 + a2, B -> b0 + b1 (BF16 each); the
     kernel forms a0b0 + a0b1 + a1b0 + a1b1 + a2b0 (a2b1 is dropped).
-    compute_type 2 (fp16, prescaled): row r of op(A) -> (hi + lo) * 2^e(r).
+    compute_type 2 (fp16, prescaled, the 256x256 / ring / three-stage /
+    pre-split kernels): row r of op(A) -> (hi + lo) * 2^e(r).
+    compute_type 3 (fp16, prescaled, the one-deep kernels): k-tile t of row r
+    -> (hi + lo) * 2^e(r, t), running exponents over ktile-wide k-tiles,
+    restarting at every k-slice of kslice columns.
     The GPU result differs from this only by FP32 accumulation rounding.
     """
     A = _check_f32(A, "A")
     B = _check_f32(B, "B")
-    if compute_type == 2:
+    if compute_type in (2, 3):
         Am = np.array(op_view(A, op_a, m, k, lda))
-        Ae = split_value_f16_rowscaled(Am)
+        Ae = split_value_f16_rowscaled(Am) if compute_type == 2 else split_value_f16_running(Am, ktile, kslice)
         Bm = op_view(B.astype(np.float64), op_b, k, n, ldb)
         out = np.zeros((ldc * n) if op_c == OP_N else (ldc * m), dtype=np.float64)
         view = _as_c_view(out, op_c, m, n, ldc)
@@ -277,6 +321,23 @@ def rel_fro_error(C, C64) -> float:
 # ---------------------------------------------------------------------------
 # Seeded generators (numpy PCG64); the same streams feed tests and bench.
 # ---------------------------------------------------------------------------
+def gen_growing_rows(rng: np.random.Generator, m, n, k, step=3, width=32, b0=12):
+    """A (m x k) and B (k x n, FP16 values), column-major, whose products are
+    balanced while A's rows grow along k: odd rows of A and the k-rows of B
+    carry 2^(step*j) and 2^(b0 - step*j) in column block j (width columns;
+    B stays an FP16 normal for step*j <= b0 + 14), even rows of A are
+    U[-1, 1]; one row of A is zero.  A split with each row's final exponent
+    loses the early blocks' low bits; running exponents keep them."""
+    j = np.arange(k) // width
+    A = rng.uniform(1.0, 2.0, (m, k)) * np.where(rng.random((m, k)) < 0.5, -1.0, 1.0)
+    A[1::2] *= np.exp2(step * j)[None, :]
+    A[0::2] = rng.uniform(-1.0, 1.0, (A[0::2].shape))
+    A[m // 2] = 0.0
+    B = (rng.standard_normal((k, n)) * np.exp2(b0 - step * j)[:, None]).astype(np.float16)
+    return (np.asfortranarray(A.astype(np.float32)).ravel(order="F"),
+            np.asfortranarray(B).ravel(order="F"))
+
+
 def gen_uniform(rng: np.random.Generator, size, lo=-1.0, hi=1.0) -> np.ndarray:
     return rng.uniform(lo, hi, size).astype(np.float32)
 

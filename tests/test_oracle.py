@@ -219,6 +219,82 @@ def test_emulation_and_planted_mistakes():
         assert oracle.rel_fro_error(wrong, ref) > 8 * bar, name
 
 
+def _running_case(rng, m, k):
+    """Rows with their maximum in various k-tiles, subnormal and zero rows."""
+    A = rng.uniform(-1.0, 1.0, (m, k)) * np.exp2(rng.integers(-40, 40, (m, 1)))
+    grow = np.exp2(np.arange(k) // 32 * rng.integers(-3, 4, (m, 1)))
+    A = (A * grow).astype(np.float32)
+    A[3] = np.float32(1e-40) * rng.uniform(-1, 1, k).astype(np.float32)   # subnormal row
+    A[5, :40] = 0.0                                                        # two zero tiles first
+    A[7] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("ktile,kslice", [(32, None), (64, None), (32, 96), (64, 128)])
+def test_running_row_exponents_properties(ktile, kslice):
+    """Running exponents: the last k-tile's equals the whole-row exponent of
+    the C oracle (finite non-zero rows, unsplit); non-decreasing within a
+    k-slice and restarting at each; every k-tile fits FP16 after scaling
+    (max |a| 2^-e < 2^15), and where the exponent rose above the previous
+    one the tile's maximum sets it (max |a| 2^-e >= 2^14)."""
+    rng = np.random.default_rng(ktile + (kslice or 0))
+    m, k = 48, 300
+    A = _running_case(rng, m, k)
+    e = oracle.running_row_exponents(A, ktile, kslice)
+    T = e.shape[1]
+    span = (kslice or k) // ktile if kslice else T
+    if not kslice:
+        live = np.abs(A).max(axis=1) > 0
+        assert np.array_equal(e[live, -1], oracle.row_exponents(A)[live])
+        assert np.all(e[~live] == -126)
+    for t in range(T):
+        tile = np.abs(A[:, t * ktile:(t + 1) * ktile]).max(axis=1).astype(np.float64)
+        assert np.all(np.ldexp(tile, -e[:, t]) < 2.0 ** 15)
+        first = t % span == 0
+        prev = np.full(m, -126) if first else e[:, t - 1]
+        assert np.all(e[:, t] >= prev)
+        rose = e[:, t] > prev
+        assert np.all(np.ldexp(tile[rose], -e[rose, t]) >= 2.0 ** 14)
+    if kslice:  # each slice alone gives the same exponents
+        for s0 in range(0, k, kslice):
+            sub = oracle.running_row_exponents(A[:, s0:s0 + kslice], ktile)
+            assert np.array_equal(sub, e[:, s0 // ktile:s0 // ktile + sub.shape[1]])
+
+
+def test_running_split_reconstructs_and_matches_whole_row():
+    """The running split reconstructs each element to 2^(e - 7) (the FP16 lo
+    part's rounding at the k-tile's scale); rows whose maximum lies in their
+    first k-tile split exactly as with the whole-row exponent."""
+    rng = np.random.default_rng(31)
+    m, k = 40, 256
+    A = _running_case(rng, m, k)
+    v = oracle.split_value_f16_running(A, 32)
+    e = np.repeat(oracle.running_row_exponents(A, 32), 32, axis=1)[:, :k]
+    assert np.all(np.abs(v - A.astype(np.float64)) <= np.ldexp(1.0, e - 7))
+    first = np.abs(A[:, :32]).max(axis=1) == np.abs(A).max(axis=1)
+    first &= np.abs(A).max(axis=1) > 0
+    assert first.sum() >= 3
+    w = oracle.split_value_f16_rowscaled(A[first])
+    assert np.array_equal(v[first], w)
+
+
+def test_growing_rows_separate_running_from_whole_row():
+    """On balanced growing rows the running-exponent emulation stays within
+    the FP32 bar of the FP64 product while the whole-row emulation does not
+    (the case the GPU test uses to tell the two apart)."""
+    rng = np.random.default_rng(8)
+    m, n, k = 96, 64, 320
+    A, B16 = oracle.gen_growing_rows(rng, m, n, k)
+    B = B16.astype(np.float32)
+    f64 = oracle.gemm_f64(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m)
+    f32 = oracle.gemm_f32(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m)
+    bar = 2.0 * oracle.rel_fro_error(f32, f64)
+    run = oracle.emulate_shgemm(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m, 3, ktile=32)
+    whole = oracle.emulate_shgemm(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m, 2)
+    assert oracle.rel_fro_error(run, f64) <= bar / 4
+    assert oracle.rel_fro_error(whole, f64) > 8 * bar
+
+
 def test_wide_exponent_generator():
     rng = np.random.default_rng(15)
     a = oracle.gen_wide_exponent(rng, 100000)

@@ -61,11 +61,26 @@ __device__ __forceinline__ void grouped_order(const GemmParams& p, int t, int& t
   tn = local / rows;
 }
 
-__device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& tm, int& tn) {
-  const int nb = p.tiles_m * p.tiles_n;
+// Dispatch index -> position in an order where each XCD owns one contiguous
+// run of the nb indices (XCD x receives dispatch indices x, x + 8, ...).
+__device__ __forceinline__ int xcd_run(int bid, int nb) {
   const int q = nb >> 3, r = nb & 7;
   const int x = bid & 7, i = bid >> 3;
-  grouped_order(p, x * q + (x < r ? x : r) + i, tm, tn);
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ void tile_coords(const GemmParams& p, int bid, int& tm, int& tn) {
+  grouped_order(p, xcd_run(bid, p.tiles_m * p.tiles_n), tm, tn);
+}
+
+// Split-K grid (tiles x slices): the whole grid in dispatch order goes
+// through the XCD runs, slice-major, so one XCD works on few slices and
+// keeps their B panel (and the A panels its tiles share) in its L2.
+__device__ __forceinline__ void tile_slice_coords(const GemmParams& p, int& tm, int& tn, int& slice) {
+  const int nt = gridDim.x;
+  const int run = xcd_run(blockIdx.x + blockIdx.y * nt, nt * gridDim.y);
+  slice = run / nt;
+  grouped_order(p, run - slice * nt, tm, tn);
 }
 
 // LDS operand layout: row-major 16-byte slots, the chunk index of each row

@@ -1058,18 +1058,20 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
 }
 
 // The single-problem kernel: blockIdx.x -> output tile (XCD-aware order);
-// with SPLITK, blockIdx.y selects the k-slice and the partial goes to
-// p.ws + slice * m * n.
+// with SPLITK the grid is tiles x slices (also XCD-aware, slice-major) and
+// the partial goes to p.ws + slice * m * n.
 template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
 __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel(const GemmParams p) {
   extern __shared__ uint4 lds[];
   int tm, tn;
-  tile_coords(p, blockIdx.x, tm, tn);
   if constexpr (SPLITK) {
-    const int kbeg = blockIdx.y * p.k_slice;
+    int slice;
+    tile_slice_coords(p, tm, tn, slice);
+    const int kbeg = slice * p.k_slice;
     shgemm_tile<OPA, OPB, OPC, MODE, CFG, 1>(p, tm, tn, kbeg, min(p.k, kbeg + p.k_slice),
-                                             p.ws + (int64_t)blockIdx.y * p.m * p.n, lds);
+                                             p.ws + (int64_t)slice * p.m * p.n, lds);
   } else {
+    tile_coords(p, blockIdx.x, tm, tn);
     shgemm_tile<OPA, OPB, OPC, MODE, CFG, 0>(p, tm, tn, 0, p.k, nullptr, lds);
   }
 }

@@ -61,6 +61,18 @@ __device__ __forceinline__ void grouped_order(const GemmParams& p, int t, int& t
   tn = local / rows;
 }
 
+// A pointer read from memory (not a kernel argument) is generic to the
+// compiler, so its accesses become flat_* instructions, and a flat load also
+// counts against the LDS wait counter.  A round trip through the global
+// address space tells the compiler it points to device memory, so they are
+// global_* instructions.
+template <class T>
+__device__ __forceinline__ T* as_global(T* ptr) {
+  using G = __attribute__((address_space(1))) T;
+  G* g = (G*)(uintptr_t)ptr;
+  return (T*)g;
+}
+
 // Dispatch index -> position in an order where each XCD owns one contiguous
 // run of the nb indices (XCD x receives dispatch indices x, x + 8, ...).
 __device__ __forceinline__ int xcd_run(int bid, int nb) {

@@ -41,7 +41,12 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_grouped_kernel(const
       if (__builtin_amdgcn_readfirstlane(g.unit_begin[mid]) <= u) lo = mid;
       else hi = mid - 1;
     }
-    const GemmParams p = load_uniform(&g.probs[lo].p);
+    GemmParams p = load_uniform(&g.probs[lo].p);
+    p.a = as_global(p.a);
+    p.b = as_global(p.b);
+    p.c = as_global(p.c);
+    p.ws = as_global(p.ws);
+    p.row_exp = as_global(p.row_exp);
     const int tiles = p.tiles_m * p.tiles_n;
     const int v = u - __builtin_amdgcn_readfirstlane(g.unit_begin[lo]);
     const int slice = v / tiles;

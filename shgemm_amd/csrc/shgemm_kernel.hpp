@@ -679,7 +679,13 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   TileF32<OPA == OP_T, BM, BK, NT> ta[PF];
   TileF16<OPB == OP_N, BN, BK, NT> tb[PF];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int tid = threadIdx.x;
+  // In the persistent grouped kernel (SPLIT == 2) this body runs once per
+  // work unit: an opaque copy of the thread index keeps the compiler from
+  // hoisting every per-lane offset out of the unit loop, where they would
+  // stay live (and spill) across all units instead of one tile.
+  if constexpr (SPLIT == 2) asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
   const int m0 = tm * BM, n0 = tn * BN;

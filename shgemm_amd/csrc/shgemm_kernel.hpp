@@ -685,7 +685,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   // hoisting every per-lane offset out of the unit loop, where they would
   // stay live (and spill) across all units instead of one tile.
   if constexpr (SPLIT == 2) asm volatile("" : "+v"(tid));
-  const int lane = tid & 63, wave = tid >> 6;
+  // the wavefront index is uniform: scalar registers for it and what derives from it
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
   const int m0 = tm * BM, n0 = tn * BN;

@@ -349,8 +349,9 @@ __global__ void __launch_bounds__(256) grouped_rowmax_kernel(const RowmaxSeg* __
   const int local = b - sg.block_begin;
   const int bx = local % sg.bx, by = local / sg.bx;
   if constexpr (OPA == OP_N)
-    rowmax_rcontig_block<true>(sg.a, sg.lda, sg.m, sg.k, sg.kchunk, sg.rq, sg.vec, sg.row_max, bx, by);
-  else rowmax_kcontig_block<true>(sg.a, sg.lda, sg.k, sg.kchunk, sg.vec, sg.row_max, bx, by);
+    rowmax_rcontig_block<true>(as_global(sg.a), sg.lda, sg.m, sg.k, sg.kchunk, sg.rq, sg.vec, as_global(sg.row_max),
+                               bx, by);
+  else rowmax_kcontig_block<true>(as_global(sg.a), sg.lda, sg.k, sg.kchunk, sg.vec, as_global(sg.row_max), bx, by);
 }
 
 __global__ void __launch_bounds__(256) grouped_reduce_kernel(const ReduceSeg* __restrict__ segs,
@@ -359,8 +360,9 @@ __global__ void __launch_bounds__(256) grouped_reduce_kernel(const ReduceSeg* __
   const int i = find_segment(begin, nseg, b);
   const ReduceSeg sg = segs[i];
   const int64_t e = (int64_t)(b - sg.block_begin) * 256 + threadIdx.x;
-  if (sg.slices > 0) splitk_reduce_elem(sg.ws, sg.slices, sg.elems, sg.inner, sg.alpha, sg.beta, sg.c, sg.ldc, e);
-  else scale_c_elem(sg.elems, sg.inner, sg.beta, sg.c, sg.ldc, e);  // k == 0 or alpha == 0
+  float* c = as_global(sg.c);  // descriptor pointers: global, not flat, accesses
+  if (sg.slices > 0) splitk_reduce_elem(as_global(sg.ws), sg.slices, sg.elems, sg.inner, sg.alpha, sg.beta, c, sg.ldc, e);
+  else scale_c_elem(sg.elems, sg.inner, sg.beta, c, sg.ldc, e);  // k == 0 or alpha == 0
 }
 
 hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,

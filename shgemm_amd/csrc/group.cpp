@@ -1,9 +1,13 @@
 // Grouped SHGEMM, host side: plans every problem of a group (tile class and
-// k-slices), lays out the descriptors, row-exponent scratch and split-K
-// partials in one device allocation, orders each tile class's work units
-// longest first, and captures the launch sequence in one HIP graph:
-//   fill (queue heads, row maxima) -> grouped row-max -> row exponents ->
-//   one persistent launch per tile class -> grouped split-K reduction.
+// k-slices), lays out the descriptors and split-K partials in one device
+// allocation, orders each tile class's work units longest first, and
+// captures the launch sequence in one HIP graph:
+//   fill (queue heads) -> one persistent launch per tile class -> grouped
+//   split-K reduction.
+// Problems of a tile class whose kernel reads whole-row exponents (prescaled
+// fp16 without running exponents) add row-max segments: the fill also clears
+// their row maxima and a grouped row-max and the row exponents run before the
+// class launches.  The shipped classes keep running exponents.
 // There is no grouped entry point in the reference; each problem follows the
 // single-call semantics of mtk::shgemm::shgemm (src/shgemm.cu:517-686).
 #include <hip/hip_runtime.h>

@@ -78,7 +78,8 @@ void register_all(KernelTable&);
 // One problem of a group as the device sees it.  p.ws is the problem's own
 // partial buffer (slice-major, m*n floats per slice) when it is k-sliced,
 // nullptr otherwise; p.row_exp points at the problem's rows of the group's
-// exponent buffer in the prescaled mode.
+// exponent buffer when its class's kernel reads whole-row exponents, nullptr
+// otherwise (classes with running exponents).
 struct GroupProblem {
   GemmParams p;
   int unit_begin;  // first work unit of the problem

@@ -258,6 +258,50 @@ def test_running_exponents_match_the_oracle(h, op_a, op_c, plan, slices, k, step
     assert oracle.rel_fro_error(got, ref) <= 2.0 * err_s
 
 
+@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (3, 2), (5, 1), (8, 1)])
+def test_non_finite_inputs_stay_in_their_rows_and_columns(h, plan, slices):
+    """A NaN or an infinity in A makes its row of C non-finite and one in B
+    its column (the FP16 residual of an infinity is NaN, as in the
+    reference's split); every other element keeps the FP64 bar, in the
+    running-exponent kernels (plans 0, 1, 3) and the whole-row ones (5, 8)."""
+    import shgemm_amd as sg
+
+    m, n, k = 300, 260, 1024
+    rng = np.random.default_rng(70 + plan)
+    A = oracle.gen_uniform(rng, m * k).reshape(k, m)   # column-major m x k: A[j, i] is element (i, j)
+    A[100, 7] = np.nan
+    A[900, 20] = np.inf
+    A[0, 33] = -np.inf
+    A = A.ravel()
+    B16 = oracle.gen_f16_normal(rng, k * n).reshape(n, k)
+    B16[3, 5] = np.inf
+    B16 = B16.ravel()
+    a_t, a_p = dev_buffer(A)
+    b_t, b_p = dev_buffer(B16)
+    c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
+    sg.force_plan(h, plan, slices)
+    try:
+        assert sg.query_plan(h, 0, 0, m, n, k, sg.fp16)[:2] == (plan, slices)
+        sg.shgemm(h, 0, 0, m, n, k, 1.0, a_p, m, b_p, k, 0.0, c_p, m, sg.fp16)
+    finally:
+        sg.force_plan(h, -1, 1)
+    got = read_buffer(c_t, 0, m * n).reshape(n, m).T
+    bad_r, bad_c = [7, 20, 33], [3]
+    assert not np.isfinite(got[bad_r, :]).any()
+    assert not np.isfinite(got[:, bad_c]).any()
+    ok_r = np.setdiff1d(np.arange(m), bad_r)
+    ok_c = np.setdiff1d(np.arange(n), bad_c)
+    B32 = B16.astype(np.float32)
+    Am = A.reshape(k, m).T.astype(np.float64)
+    Bm = B32.reshape(n, k).T.astype(np.float64)
+    ref = Am[ok_r] @ Bm[:, ok_c]
+    Cs = run_sgemm(h, 0, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, np.zeros(m * n, np.float32), m).reshape(n, m).T
+    err = oracle.rel_fro_error(got[np.ix_(ok_r, ok_c)], ref)
+    err_s = oracle.rel_fro_error(Cs[np.ix_(ok_r, ok_c)], ref)
+    assert np.isfinite(got[np.ix_(ok_r, ok_c)]).all()
+    assert err <= 2.0 * err_s, (plan, err, err_s)
+
+
 def test_kernel_timing_counts_main_kernel_launches(h):
     """enable_kernel_timing brackets the main GEMM kernel of the next N calls;
     each query returns the launches since the previous one."""

@@ -66,7 +66,9 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_grouped_kernel(const
 
 // Tile classes of the grouped kernel: the small tile, the 128 x 128 tile and
 // the mode's large tile (the single-call P0, SK and P1 configurations, so a
-// problem run in a group gives the same bits as the same plan run alone).
+// problem run in a group gives the same bits as the same plan run alone; the
+// k-sliced small-tile plan differs from P0 only in its prefetch depth, which
+// does not change the arithmetic).
 template <int MODE>
 struct GroupCfg {
   using C0 = typename ModeCfg<MODE>::P0;

@@ -443,12 +443,14 @@ struct TileCfg {
   static constexpr int stage_chunks() {
     return (MODE == MODE_BF16 ? 3 : 2) * BM * KC + (MODE == MODE_BF16 ? 2 : 1) * BN * KC;
   }
-  // Prescaled fp16 with one k-tile of prefetch: the tile keeps a running
-  // exponent per row itself (RunState in LDS after the two stages) instead of
-  // reading exponents from a separate pass over A.
+  // Prescaled fp16 on wavefront tiles up to 64 rows, without the skewed
+  // schedule: the tile keeps a running exponent per row itself (RunState in
+  // LDS after the two stages) instead of reading exponents from a separate
+  // pass over A.  (The 128-row wavefront tiles hold their accumulators in
+  // AGPRs and have no registers left for it.)
   template <int MODE>
   static constexpr bool running() {
-    return MODE == MODE_FP16S && PF == 1;
+    return MODE == MODE_FP16S && !IL && WM <= 64;
   }
   template <int MODE>
   static constexpr size_t lds_bytes() {
@@ -473,6 +475,7 @@ template <int MODE>
 struct ModeCfg {
   using P1 = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
+  using P0SK = P0;  // the small tile of the k-sliced plan
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 1, 1>;
   static constexpr bool kHasP2 = (MODE == MODE_BF16);
@@ -484,7 +487,12 @@ struct ModeCfg {
 template <>
 struct ModeCfg<MODE_FP16S> {
   using P1 = TileCfg<128, 256, 32, 64, 64, 1, 4>;  // 8 wavefronts, 2 workgroups (4 wavefronts) per SIMD
-  using P0 = TileCfg<64, 64, 64, 32, 32, 1, 4>;
+  // The small tile keeps two k-tiles of loads in flight (512^3 to 4096^3 on
+  // this tile: 1-3% faster, level at 1024^3); k-sliced it keeps one (the
+  // 128 x 128 x 2^20 slices ran 8% slower with two).  Same arithmetic, same
+  // bits.
+  using P0 = TileCfg<64, 64, 64, 32, 32, 2, 3>;
+  using P0SK = TileCfg<64, 64, 64, 32, 32, 1, 4>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 3>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 2, 1, true>;
   static constexpr bool kHasP2 = true;
@@ -719,7 +727,6 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   RS* rs = nullptr;
   int ecur[MI::PER];
   if constexpr (RUN) {
-    static_assert(PF == 1, "running exponents use one register set");
     rs = reinterpret_cast<RS*>(lds + 2 * STAGE);
     for (int r = tid; r < BM; r += NT) {
       rs->rmax[0][r] = 0u;
@@ -730,7 +737,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
 #pragma unroll
     for (int q = 0; q < MI::PER; ++q) {
       ecur[q] = kFloor;
-      ta[0].scale[q] = __builtin_ldexpf(1.0f, -kFloor);
+#pragma unroll
+      for (int i = 0; i < PF; ++i) ta[i].scale[q] = __builtin_ldexpf(1.0f, -kFloor);
     }
     __syncthreads();
   } else if constexpr (SCALED) {
@@ -770,13 +778,15 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       tb[set].store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
     }
   };
-  // RUN, before k-tile t (in register set 0) is stored: flag the chunks that
-  // need a larger exponent (par_c == t & 1, an int or an integral_constant)
-  auto run_check = [&](int t, auto par_c) {
+  // RUN, before k-tile t (in register set `set_c`) is stored: flag the
+  // chunks that need a larger exponent (par_c == t & 1, an int or an
+  // integral_constant)
+  auto run_check = [&](auto set_c, int t, auto par_c) {
+    constexpr int set = decltype(set_c)::value;
     const int par = static_cast<int>(par_c);
 #pragma unroll
     for (int q = 0; q < MI::PER; ++q) {
-      const uint32_t b = ta[0].item_max(q);
+      const uint32_t b = ta[set].item_max(q);
       const int et = b == 0u ? kFloor : exponent_of_max(b);
       if (et > ecur[q]) {
         int r, c;
@@ -786,10 +796,11 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       }
     }
   };
-  // RUN, after the barrier that ends k-tile t's stores into `stage`: whether
-  // some row's exponent rose at t (uniform); if so the new exponents, the
-  // factors and the k-tile split again (contains a barrier)
-  auto run_settle = [&](int t, auto par_c, int stage) -> bool {
+  // RUN, after the barrier that ends k-tile t's stores (from register set
+  // `set_c`) into `stage`: whether some row's exponent rose at t (uniform);
+  // if so the new exponents (every register set's scales), the factors and
+  // the k-tile split again (contains a barrier)
+  auto run_settle = [&](auto set_c, int t, auto par_c, int stage) -> bool {
     const int par = static_cast<int>(par_c);
     if (__builtin_amdgcn_readfirstlane(rs->flag[par]) != t) return false;
 #pragma unroll
@@ -804,9 +815,10 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
         rs->erow[r] = en;
       }
       ecur[q] = en;
-      ta[0].scale[q] = __builtin_ldexpf(1.0f, -en);
+#pragma unroll
+      for (int i = 0; i < PF; ++i) ta[i].scale[q] = __builtin_ldexpf(1.0f, -en);
     }
-    sstore(S0{}, stage);
+    sstore(set_c, stage);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MI::PER; ++q) {
@@ -857,12 +869,12 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     bool rose = false;  // RUN: some row's exponent rose at the next k-tile to multiply
     if (ktiles > 0) {
       gload(S0{}, 0);
-      if constexpr (RUN) run_check(0, P0{});
+      if constexpr (RUN) run_check(S0{}, 0, P0{});
       sstore(S0{}, 0);
     }
     __syncthreads();
     if constexpr (RUN) {
-      if (ktiles > 0) run_settle(0, P0{}, 0);  // nothing accumulated yet: no rescale
+      if (ktiles > 0) run_settle(S0{}, 0, P0{}, 0);  // nothing accumulated yet: no rescale
     }
     int kt = 0;
     // Interior tiles while the next k-tile is full: only the unchecked loads,
@@ -886,11 +898,11 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       compute(lds + cur * STAGE);
       if constexpr (RUN) {
         __builtin_amdgcn_sched_barrier(0);  // the exponent checks wait for the loads after the MFMAs
-        run_check(kt + 1, NX{});
+        run_check(S0{}, kt + 1, NX{});
       }
       sstore(S0{}, cur ^ 1);
       __syncthreads();
-      if constexpr (RUN) rose = run_settle(kt + 1, NX{}, cur ^ 1);
+      if constexpr (RUN) rose = run_settle(S0{}, kt + 1, NX{}, cur ^ 1);
       ++kt;
     };
     auto fast_loop = [&](auto bload) {
@@ -913,19 +925,28 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       }
       compute(lds + cur * STAGE);
       if (kt + 1 < ktiles) {
-        if constexpr (RUN) run_check(kt + 1, cur ^ 1);
+        if constexpr (RUN) run_check(S0{}, kt + 1, cur ^ 1);
         sstore(S0{}, cur ^ 1);
       }
       __syncthreads();
       if constexpr (RUN) {
-        if (kt + 1 < ktiles) rose = run_settle(kt + 1, cur ^ 1, cur ^ 1);
+        if (kt + 1 < ktiles) rose = run_settle(S0{}, kt + 1, cur ^ 1, cur ^ 1);
       }
     }
   } else {
+    using Q0 = std::integral_constant<int, 0>;
+    using Q1 = std::integral_constant<int, 1>;
+    bool rose = false;  // RUN: some row's exponent rose at the next k-tile to multiply
     if (ktiles > 0) gload(S0{}, 0);
     if (ktiles > 1) gload(S1{}, 1);
-    if (ktiles > 0) sstore(S0{}, 0);
+    if (ktiles > 0) {
+      if constexpr (RUN) run_check(S0{}, 0, Q0{});
+      sstore(S0{}, 0);
+    }
     __syncthreads();
+    if constexpr (RUN) {
+      if (ktiles > 0) run_settle(S0{}, 0, Q0{}, 0);  // nothing accumulated yet: no rescale
+    }
     // Register set s holds tile kt+1 (odd/even alternate), stage kt&1 holds tile kt.
     int kt = 0;
     // Interior pairs while tiles kt+2 and kt+3 are full: unchecked loads only
@@ -994,32 +1015,68 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
         }
       }
     } else if (a_full && b_full) {
+      // (running exponents: tile kt+1, in set 1, has parity 1; tile kt+2,
+      // in set 0, parity 0)
       for (; kt + 3 < kfull; kt += 2) {
         const int k2 = kbeg + (kt + 2) * BK;
         ta[0].load_fast(A, p.lda, m0, k2, tid);
         tb[0].load_fast(B, p.ldb, n0, k2, tid);
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (RUN) {
+          if (rose) run_rescale();
+        }
         compute(lds);
+        if constexpr (RUN) {
+          __builtin_amdgcn_sched_barrier(0);
+          run_check(S1{}, kt + 1, Q1{});
+        }
         sstore(S1{}, 1);
         __syncthreads();
+        if constexpr (RUN) rose = run_settle(S1{}, kt + 1, Q1{}, 1);
         ta[1].load_fast(A, p.lda, m0, k2 + BK, tid);
         tb[1].load_fast(B, p.ldb, n0, k2 + BK, tid);
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (RUN) {
+          if (rose) run_rescale();
+        }
         compute(lds + STAGE);
+        if constexpr (RUN) {
+          __builtin_amdgcn_sched_barrier(0);
+          run_check(S0{}, kt + 2, Q0{});
+        }
         sstore(S0{}, 0);
         __syncthreads();
+        if constexpr (RUN) rose = run_settle(S0{}, kt + 2, Q0{}, 0);
       }
     }
     for (; kt < ktiles; kt += 2) {
       if (kt + 2 < ktiles) gload(S0{}, kt + 2);
+      if constexpr (RUN) {
+        if (rose) run_rescale();
+      }
       compute(lds);
-      if (kt + 1 < ktiles) sstore(S1{}, 1);
+      if (kt + 1 < ktiles) {
+        if constexpr (RUN) run_check(S1{}, kt + 1, Q1{});
+        sstore(S1{}, 1);
+      }
       __syncthreads();
+      if constexpr (RUN) {
+        if (kt + 1 < ktiles) rose = run_settle(S1{}, kt + 1, Q1{}, 1);
+      }
       if (kt + 1 >= ktiles) break;
       if (kt + 3 < ktiles) gload(S1{}, kt + 3);
+      if constexpr (RUN) {
+        if (rose) run_rescale();
+      }
       compute(lds + STAGE);
-      if (kt + 2 < ktiles) sstore(S0{}, 0);
+      if (kt + 2 < ktiles) {
+        if constexpr (RUN) run_check(S0{}, kt + 2, Q0{});
+        sstore(S0{}, 0);
+      }
       __syncthreads();
+      if constexpr (RUN) {
+        if (kt + 2 < ktiles) rose = run_settle(S0{}, kt + 2, Q0{}, 0);
+      }
     }
   }
 

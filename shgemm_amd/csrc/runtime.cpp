@@ -491,8 +491,9 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
     }
   }
   const shgemm_amd::KernelEntry& e = plan.entry != nullptr ? *plan.entry : tab[plan.variant];
-  // Prescaled fp16: the kernels with one k-tile of prefetch keep running row
-  // exponents themselves; the others read them from a pass over A.
+  // Prescaled fp16: the kernels on wavefront tiles of up to 64 rows (P0, P1,
+  // the split-K plans) keep running row exponents themselves; the 256 x 256
+  // family and the three-stage kernel read them from a pass over A.
   if (scaled && e.needs_row_exp) {
     if (st->row_cap < m) {
       // Grow the prescale scratch; earlier work on this handle may still use it.

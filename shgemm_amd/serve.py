@@ -88,10 +88,11 @@ class GpuLanes:
     def reserve(self, max_m: int, max_k: int = 0, max_l: int = 0) -> None:
         """Prepare every lane for requests up to (max_m, max_k, max_l) before
         serving, so no device allocation lands inside a request's latency:
-        grow each handle's per-row scratch to max_m rows (growth synchronises
-        the lane's stream) with a 1-column product, and, when max_k and max_l
-        are given, leave room for two requests' Omega and Y in the lane
-        stream's pool of the caching allocator (pools are per stream)."""
+        run a 1-column product on each handle (its first call; a handle grows
+        its per-row scratch there if the plan reads whole-row exponents --
+        growth synchronises the lane's stream), and, when max_k and max_l are
+        given, leave room for two requests' Omega and Y in the lane stream's
+        pool of the caching allocator (pools are per stream)."""
         torch = self.torch
         a = torch.zeros(max_m, device=self.device)
         b = torch.ones(1, dtype=torch.float16, device=self.device)

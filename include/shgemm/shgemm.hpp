@@ -128,11 +128,11 @@ void sgemm_fp32(
 // Per-row power-of-two prescaling of the FP32 operand for the fp16 compute
 // type.  On by default (MTK_SHGEMM_EXP_SCALING=0 turns it off at create, which
 // gives the reference's fp16 mode with FP16's exponent range).  When on, the
-// split runs on A * 2^-e and the row of C is scaled back exactly: the
-// one-deep kernels (plans 0-4, grouped) keep a running exponent per row over
-// their k-range (rescaling their accumulator rows by exact powers of two when
-// it rises); the 256x256, ring, three-stage and pre-split kernels (plans 5-8)
-// take each row's whole-row exponent from one extra pass over A.
+// split runs on A * 2^-e and the row of C is scaled back exactly: the small,
+// large and split-K tiles (plans 0-4, grouped) keep a running exponent per
+// row over their k-range (rescaling their accumulator rows by exact powers of
+// two when it rises); the 256x256, ring, three-stage and pre-split kernels
+// (plans 5-8) take each row's whole-row exponent from one extra pass over A.
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 
 // Device sharing (no counterpart in the reference): on a handle whose calls

@@ -13,7 +13,7 @@ Contents
   emulate_shgemm             FP64 evaluation of the split representation the
                              GPU multiplies: alpha*(op(hi) + 2^-11 op(lo)) op(B) + beta*C0,
                              and the prescaled forms (whole-row exponents,
-                             running_row_exponents of the one-deep kernels)
+                             running_row_exponents of the P0/P1/split-K kernels)
   rel_fro_error              ||C - C64||_F / ||C64||_F (the error metric)
   philox4x32_10, philox_words, gaussian
                              the sketch generator (Philox4x32-10 + Box-Muller),
@@ -166,7 +166,7 @@ def split_value_f16_rowscaled(Am: np.ndarray) -> np.ndarray:
 
 
 def running_row_exponents(Am: np.ndarray, ktile: int, kslice: int | None = None) -> np.ndarray:
-    """Running row exponents of the one-deep kernels (the prescaled fp16 P0,
+    """Running row exponents of the 64-row-wavefront kernels (the prescaled fp16 P0,
     P1, split-K and grouped tiles): an m x T array, T = number of ktile-wide
     k-tiles, whose entry (r, t) is the exponent k-tile t of row r is split
     with: max(previous entry, ilogb(max |tile|) - 14), clamped below at -126,
@@ -239,7 +239,7 @@ def emulate_shgemm(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, l
     kernel forms a0b0 + a0b1 + a1b0 + a1b1 + a2b0 (a2b1 is dropped).
     compute_type 2 (fp16, prescaled, the 256x256 / ring / three-stage /
     pre-split kernels): row r of op(A) -> (hi + lo) * 2^e(r).
-    compute_type 3 (fp16, prescaled, the one-deep kernels): k-tile t of row r
+    compute_type 3 (fp16, prescaled, the P0 / P1 / split-K kernels): k-tile t of row r
     -> (hi + lo) * 2^e(r, t), running exponents over ktile-wide k-tiles,
     restarting at every k-slice of kslice columns.
     The GPU result differs from this only by FP32 accumulation rounding.

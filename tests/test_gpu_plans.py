@@ -210,14 +210,14 @@ def test_running_exponents_follow_growing_rows(h, op_a, op_c, plan, slices):
     assert np.all(r <= 2.0 * r_s + 2.0 ** -22), (plan, slices, float(r.max()), float(r_s.max()))
 
 
-KTILE = {0: 64, 1: 32, 2: 32, 3: 32, 4: 64}  # k-tile depth of each one-deep plan's kernel
+KTILE = {0: 64, 1: 32, 2: 32, 3: 32, 4: 64}  # k-tile depth of each running-exponent plan's kernel
 
 
 @pytest.mark.parametrize("op_a,op_c", [(0, 0), (1, 1)])
 @pytest.mark.parametrize("plan,slices,k,step,width", [(0, 1, 320, 3, 32), (1, 1, 320, 3, 32), (2, 2, 1024, 4, 128),
                                                        (3, 2, 1024, 4, 128), (4, 2, 1024, 4, 128)])
 def test_running_exponents_match_the_oracle(h, op_a, op_c, plan, slices, k, step, width):
-    """Balanced growing rows (oracle.gen_growing_rows): the one-deep kernels
+    """Balanced growing rows (oracle.gen_growing_rows): the running-exponent kernels
     agree with the oracle's running-exponent emulation (k-tiles of their
     depth, restarting at each k-slice) to FP32 accumulation accuracy, and not
     with the whole-row emulation, which misses the FP64 product by far more

@@ -9,6 +9,8 @@ Bars:
     sequence give identical results;
   * with the plans the library picks itself, each problem's error against an
     FP64 product is within 2x of the library's FP32 SGEMM on the same inputs;
+  * on rows that grow along k (running exponents rising at later k-tiles)
+    the grouped result still equals the single call bit for bit;
   * invalid problems are rejected before anything is enqueued.
 """
 from __future__ import annotations
@@ -214,3 +216,50 @@ def test_group_rejects_invalid(h):
     with pytest.raises(sg.ShgemmError) as ei:
         sg.Group(h, 0, 0, [], 0)
     assert ei.value.status == sg.STATUS_EINVAL
+
+
+class GrowingProb(Prob):
+    """A problem on oracle.gen_growing_rows operands (op_n / op_n / op_n):
+    rows of A that grow along k, so running exponents rise at later k-tiles."""
+
+    def __init__(self, rng, m, n, k, step, width):
+        self.m, self.n, self.k, self.alpha, self.beta = m, n, k, 1.0, 0.0
+        self.lda, self.ldb, self.ldc = m, k, m
+        self.A, self.B16 = oracle.gen_growing_rows(rng, m, n, k, step=step, width=width)
+        self.C0 = np.zeros(m * n, np.float32)
+        self.a_t, self.a_p = dev_buffer(self.A)
+        self.b_t, self.b_p = dev_buffer(self.B16)
+        self.c_t, self.c_p = dev_buffer(self.C0)
+
+
+def test_group_with_growing_rows_matches_single_calls(h):
+    """Running exponents inside the persistent kernel (re-split and rescale
+    when a row's maximum grows, state re-initialised per work unit): every
+    problem equals the same plan run alone, bit for bit, and stays within 2x
+    of the library SGEMM's FP64 error."""
+    import shgemm_amd as sg
+    from tests.gpu_util import run_sgemm
+
+    rng = np.random.default_rng(4242)
+    specs = [  # m, n, k, step, width, tile class, slices
+        (300, 200, 320, 3, 32, 0, 1), (301, 150, 320, 3, 32, 1, 1), (256, 256, 320, 3, 32, 2, 1),
+        (190, 64, 1024, 4, 128, 0, 2), (200, 128, 1024, 4, 128, 1, 2), (128, 256, 1024, 4, 128, 2, 2),
+    ]
+    probs = [GrowingProb(rng, m, n, k, st, w) for m, n, k, st, w, _, _ in specs]
+    classes = [s[5] for s in specs]
+    slices = [s[6] for s in specs]
+    with sg.Group(h, 0, 0, [p.tup() for p in probs], sg.fp16, 0, classes, slices) as g:
+        got_cls, got_sl, _ = g.plan()
+        g.run()
+        grouped = [p.c() for p in probs]
+    for i, (p, s) in enumerate(zip(probs, specs)):
+        assert (got_cls[i], got_sl[i]) == (s[5], s[6])
+        unsplit, split = CLASS_PLAN[s[5]]
+        alone = _single(h, p, 0, 0, 0, FP16, (split, s[6]) if s[6] > 1 else (unsplit, 1))
+        assert np.array_equal(grouped[i].view(np.uint32), alone.view(np.uint32)), f"problem {i}"
+        B32 = p.B16.astype(np.float32)
+        ref = p.A.reshape(p.k, p.m).T.astype(np.float64) @ B32.reshape(p.n, p.k).T.astype(np.float64)
+        Cs = run_sgemm(h, 0, 0, 0, p.m, p.n, p.k, 1.0, p.A, p.m, B32, p.k, 0.0, np.zeros(p.m * p.n, np.float32), p.m)
+        err = oracle.rel_fro_error(grouped[i].reshape(p.n, p.m).T, ref)
+        err_s = oracle.rel_fro_error(Cs.reshape(p.n, p.m).T, ref)
+        assert err <= 2.0 * err_s, (i, err, err_s)

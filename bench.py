@@ -323,12 +323,15 @@ def bench_serve(args):
 
 
 def _time_ms(fn, iters, stream):
-    """Median per-call milliseconds over 3 batches of `iters` calls (HIP events on `stream`)."""
+    """Median per-call milliseconds over 3 batches of `iters` calls (HIP events
+    on `stream`), after `iters` untimed calls that bring the GPU out of the
+    idle it falls into while the host computes the previous line's errors."""
     import statistics
 
     import torch
 
-    fn()
+    for _ in range(iters):
+        fn()
     torch.cuda.synchronize()
     out = []
     for _ in range(3):

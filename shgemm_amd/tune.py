@@ -37,9 +37,9 @@ MODES = {  # plan-table mode -> (compute type, exponent scaling)
 }
 OPS = {"N": sg.op_n, "T": sg.op_t}
 
-SINGLE_M = [128, 512, 1024, 2048, 8192, 16384]
-SINGLE_N = [64, 128, 256, 1024, 4096, 8192, 16384]
-SINGLE_K = [256, 1024, 4096, 16384, 65536, 262144, 1048576]
+SINGLE_M = [128, 512, 1024, 2048, 4096, 8192, 16384]
+SINGLE_N = [64, 128, 256, 1024, 2048, 4096, 8192, 16384]
+SINGLE_K = [256, 1024, 2048, 4096, 16384, 65536, 262144, 1048576]
 QUICK_M = [128, 2048, 16384]
 QUICK_N = [64, 1024, 16384]
 QUICK_K = [1024, 16384, 1048576]

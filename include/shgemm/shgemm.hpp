@@ -132,7 +132,8 @@ void sgemm_fp32(
 // large and split-K tiles (plans 0-4, grouped) keep a running exponent per
 // row over their k-range (rescaling their accumulator rows by exact powers of
 // two when it rises); the 256x256, ring, three-stage and pre-split kernels
-// (plans 5-8) take each row's whole-row exponent from one extra pass over A.
+// and the small tile of plan 9 (plans 5-9) take each row's whole-row
+// exponent from one extra pass over A.
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on);
 
 // Device sharing (no counterpart in the reference): on a handle whose calls
@@ -156,11 +157,20 @@ void set_shared_device(shgemmHandle_t& handle, const bool on);
 //   of the kernel and one packs B the same way (4 bytes per element of A
 //   and 2 per element of B, in a handle buffer grown on first use), then
 //   the 256x256 tile runs without a split (prescaled fp16, k a multiple of
-//   64; otherwise it runs as 5).
+//   64; otherwise it runs as 5), 9 the small tile on whole-row exponents
+//   from a pass over A instead of running ones (prescaled fp16; otherwise
+//   it runs as 0).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.
-enum { plan_variant_p2 = 5, plan_variant_p2r = 6, plan_variant_bd = 7, plan_variant_ps = 8, num_plan_variants = 9 };
+enum {
+  plan_variant_p2 = 5,
+  plan_variant_p2r = 6,
+  plan_variant_bd = 7,
+  plan_variant_ps = 8,
+  plan_variant_p0w = 9,
+  num_plan_variants = 10
+};
 enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
 struct plan_t {
   int variant = 1;

@@ -89,13 +89,15 @@ int shgemm_amd_set_shared_device(shgemm_amd_handle* h, int on);
  * A and B, 7 large tile with the three-stage direct-to-LDS B pipeline,
  * 8 the 256x256 tile on a pre-split A (A split and B packed once per call
  * into a handle buffer of 4 bytes per element of A and 2 per element of B;
- * prescaled fp16, k % 64 == 0)
- * (5-8 run as the nearest applicable variant where they do not apply).
+ * prescaled fp16, k % 64 == 0), 9 the small tile on whole-row exponents
+ * from a pass over A (prescaled fp16)
+ * (5-9 run as the nearest applicable variant where they do not apply).
  * Plan sources (query): 0 cost model, 1 plan table, 2 forced, 3 fixed level. */
 #define SHGEMM_AMD_PLAN_P2 5
 #define SHGEMM_AMD_PLAN_P2R 6
 #define SHGEMM_AMD_PLAN_BD 7
 #define SHGEMM_AMD_PLAN_PS 8
+#define SHGEMM_AMD_PLAN_P0W 9
 /* Force variant/slices for every later call on h (variant < 0 clears). */
 int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
 /* Kernel timing (no counterpart in the reference; for profiling): HIP events

@@ -8,9 +8,8 @@
 namespace shgemm_amd {
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
-inline void put_shgemm(KernelTable& t, int variant) {
+inline void fill_shgemm(KernelEntry& e) {
   static_assert(CFG::template lds_bytes<MODE>() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
-  KernelEntry& e = t.shgemm[OPA][OPB][OPC][MODE][variant];
   e.fn = reinterpret_cast<const void*>(&shgemm_kernel<OPA, OPB, OPC, MODE, CFG, SPLITK>);
   e.lds = CFG::template lds_bytes<MODE>();
   e.bm = CFG::BM;
@@ -20,6 +19,11 @@ inline void put_shgemm(KernelTable& t, int variant) {
   e.threads = CFG::NT;
   e.wgs_per_cu = (4 * CFG::OCC) / CFG::WAVES > 0 ? (4 * CFG::OCC) / CFG::WAVES : 1;
   e.needs_row_exp = MODE == MODE_FP16S && !CFG::template running<MODE>();
+}
+
+template <int OPA, int OPB, int OPC, int MODE, class CFG, bool SPLITK>
+inline void put_shgemm(KernelTable& t, int variant) {
+  fill_shgemm<OPA, OPB, OPC, MODE, CFG, SPLITK>(t.shgemm[OPA][OPB][OPC][MODE][variant]);
 }
 
 template <int OPA, int OPB, int OPC, int MODE, class CFG>
@@ -64,6 +68,7 @@ inline void put_shgemm_opc(KernelTable& t) {
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, true>(t, V_P1SK);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0SK, true>(t, V_P0SK);
   if constexpr (C::kHasP2) put_shgemm<OPA, OPB, OPC, MODE, typename C::P2, false>(t, V_P2);
+  if constexpr (MODE == MODE_FP16S) fill_shgemm<OPA, OPB, OPC, MODE, typename C::P0W, false>(t.shgemm_p0w[OPA][OPB][OPC]);
   if constexpr (MODE == MODE_FP16S && OPA == OP_N && OPB == OP_N) put_ring<OPC, typename C::P2>(t);
 }
 

@@ -48,6 +48,8 @@ struct KernelTable {
   KernelEntry shgemm_bd[2][2];
   // pre-split prescaled fp16 kernel (A split once per call, op_b = n): [op_c]
   KernelEntry shgemm_ps[2];
+  // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
+  KernelEntry shgemm_p0w[2][2][2];
   // persistent grouped kernels: [op_a][op_b][op_c][mode][tile class]
   KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,

@@ -167,6 +167,13 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
     pl.variant = shgemm_amd::V_P1;
     pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
     if (pl.entry != nullptr) pl.note = " deep-prefetch";
+  } else if (requested == plan_variant_p0w) {
+    pl.variant = shgemm_amd::V_P0;
+    const auto& e = st->table.shgemm_p0w[oa][ob][oc];
+    if (mode == shgemm_amd::MODE_FP16S && e.fn != nullptr) {
+      pl.entry = &e;
+      pl.note = " whole-row exponents";
+    }
   } else if (requested >= 0 && requested < shgemm_amd::kNumVariants && tab[requested].fn != nullptr) {
     pl.variant = requested;
     if (shgemm_amd::variant_is_split(requested)) {
@@ -412,7 +419,9 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
                               (int64_t)k, true, true, lda, ldb);
   plan_t out;
   out.variant = pl.presplit ? plan_variant_ps
-                : (pl.entry != nullptr && pl.requested == plan_variant_bd ? plan_variant_bd : pl.variant);
+                : (pl.entry != nullptr && (pl.requested == plan_variant_bd || pl.requested == plan_variant_p0w)
+                       ? pl.requested
+                       : pl.variant);
   out.slices = pl.slices;
   out.source = pl.source;
   return out;

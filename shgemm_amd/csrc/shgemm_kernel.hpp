@@ -421,7 +421,7 @@ struct RunState {
 // ---------------------------------------------------------------------------
 // Tile configuration
 // ---------------------------------------------------------------------------
-template <int BM_, int BN_, int BK_, int WM_, int WN_, int PF_, int OCC_, bool IL_ = false>
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int PF_, int OCC_, bool IL_ = false, bool RUN_ = true>
 struct TileCfg {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   // PF: k-tiles of global loads in flight ahead of the one being multiplied
@@ -447,10 +447,11 @@ struct TileCfg {
   // schedule: the tile keeps a running exponent per row itself (RunState in
   // LDS after the two stages) instead of reading exponents from a separate
   // pass over A.  (The 128-row wavefront tiles hold their accumulators in
-  // AGPRs and have no registers left for it.)
+  // AGPRs and have no registers left for it.)  RUN_ = false keeps whole-row
+  // exponents on a tile that could run them.
   template <int MODE>
   static constexpr bool running() {
-    return MODE == MODE_FP16S && !IL && WM <= 64;
+    return RUN_ && MODE == MODE_FP16S && !IL && WM <= 64;
   }
   template <int MODE>
   static constexpr size_t lds_bytes() {
@@ -476,6 +477,7 @@ struct ModeCfg {
   using P1 = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P0 = TileCfg<64, 64, 64, 32, 32, 1, 2>;
   using P0SK = P0;  // the small tile of the k-sliced plan
+  using P0W = P0;   // the small tile on whole-row exponents (prescaled mode only)
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 1, 1>;
   static constexpr bool kHasP2 = (MODE == MODE_BF16);
@@ -493,6 +495,10 @@ struct ModeCfg<MODE_FP16S> {
   // bits.
   using P0 = TileCfg<64, 64, 64, 32, 32, 2, 3>;
   using P0SK = TileCfg<64, 64, 64, 32, 32, 1, 4>;
+  // The small tile reading whole-row exponents from the row pass (plan 9):
+  // the running check costs about 0.2 us per k-tile when each SIMD holds one
+  // wavefront, which outweighs the pass for long k.
+  using P0W = TileCfg<64, 64, 64, 32, 32, 2, 3, false, false>;
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 3>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 2, 1, true>;
   static constexpr bool kHasP2 = true;

@@ -111,8 +111,8 @@ def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
         B = torch.randn(k * n, device="cuda").half()
         C = torch.empty(m * n, device="cuda")
         seen, best = set(), None
-        candidates = [(0, 1), (1, 1), (5, 1), (6, 1), (7, 1), (8, 1)] + [(v, s) for v in (2, 3, 4)
-                                                                  for s in (2, 4, 8, 16, 32, 64, 128, 256)]
+        candidates = [(v, 1) for v in (0, 1, 5, 6, 7, 8, 9)] + [(v, s) for v in (2, 3, 4)
+                                                                for s in (2, 4, 8, 16, 32, 64, 128, 256)]
         for v, s in candidates:
             sg.force_plan(h, v, s)
             eff = sg.query_plan(h, oa, ob, m, n, k, ct, oc)[:2]

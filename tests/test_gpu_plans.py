@@ -67,12 +67,12 @@ def test_bad_plan_table_is_rejected(h, tmp_path):
     import shgemm_amd as sg
 
     bad = tmp_path / "bad.tsv"
-    bad.write_text("single fp16s N N N 10 10 10 9 1\n")  # variant 9 does not exist
+    bad.write_text("single fp16s N N N 10 10 10 10 1\n")  # variant 10 does not exist
     with pytest.raises(sg.ShgemmError) as ei:
         sg.load_plans(h, bad)
     assert ei.value.status == sg.STATUS_EINVAL and "bad.tsv:1" in str(ei.value)
     with pytest.raises(sg.ShgemmError):
-        sg.force_plan(h, 9, 1)
+        sg.force_plan(h, 10, 1)
 
 
 def test_table_plan_runs_named_kernel(h, tmp_path):
@@ -124,7 +124,8 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
     odd sizes exercise every edge path.  With per-row prescaling the large
     tile keeps running row exponents (its k-tiles are scaled by the maximum
     so far, not by the row's final maximum): plans 5-7 still agree bitwise
-    and plan 1 is held to the FP64 bar (2x the library SGEMM's error)."""
+    and plan 1 is held to the FP64 bar (2x the library SGEMM's error).  The
+    small tile on whole-row exponents (plan 9) joins plans 5-7 there."""
     import shgemm_amd as sg
 
     op_a, op_b, op_c = ops
@@ -139,7 +140,7 @@ def test_large_tiles_agree_bitwise(h, ct, scaling, ops):
         c_t, c_p = dev_buffer(np.zeros(ldc * (n if op_c == 0 else m), np.float32))
         sg.set_exponent_scaling(h, scaling)
         outs = []
-        for plan in (1, 5, 6, 7):
+        for plan in (1, 5, 6, 7) + ((9,) if running else ()):
             sg.force_plan(h, plan, 1)
             sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, ct, op_c)
             outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())
@@ -258,12 +259,12 @@ def test_running_exponents_match_the_oracle(h, op_a, op_c, plan, slices, k, step
     assert oracle.rel_fro_error(got, ref) <= 2.0 * err_s
 
 
-@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (3, 2), (5, 1), (8, 1)])
+@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (3, 2), (5, 1), (8, 1), (9, 1)])
 def test_non_finite_inputs_stay_in_their_rows_and_columns(h, plan, slices):
     """A NaN or an infinity in A makes its row of C non-finite and one in B
     its column (the FP16 residual of an infinity is NaN, as in the
     reference's split); every other element keeps the FP64 bar, in the
-    running-exponent kernels (plans 0, 1, 3) and the whole-row ones (5, 8)."""
+    running-exponent kernels (plans 0, 1, 3) and the whole-row ones (5, 8, 9)."""
     import shgemm_amd as sg
 
     m, n, k = 300, 260, 1024
@@ -375,6 +376,18 @@ def test_presplit_plan_falls_back_where_it_does_not_apply(h):
     sg.force_plan(h, -1, 1)
 
 
+def test_whole_row_small_tile_falls_back_outside_prescaled_fp16(h):
+    import shgemm_amd as sg
+
+    sg.force_plan(h, 9, 1)
+    assert sg.query_plan(h, 1, 1, 300, 200, 1000, sg.fp16, 1)[0] == 9
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.tf32)[0] == 0   # BF16 split
+    sg.set_exponent_scaling(h, False)
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[0] == 0   # two-accumulator mode
+    sg.set_exponent_scaling(h, True)
+    sg.force_plan(h, -1, 1)
+
+
 def test_shared_device_avoids_whole_cu_plans(h):
     import shgemm_amd as sg
 
@@ -384,6 +397,8 @@ def test_shared_device_avoids_whole_cu_plans(h):
     for v in (5, 6, 8):
         sg.force_plan(h, v, 1)
         assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] == 1
+    sg.force_plan(h, 9, 1)  # the small tile holds no whole CU
+    assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] == 9
     sg.force_plan(h, -1, 1)
     assert sg.query_plan(h, 0, 0, 16384, 16384, 16384, sg.fp16)[0] not in (5, 6, 8)
     sg.load_plans(h, None)  # cost model

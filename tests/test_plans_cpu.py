@@ -24,7 +24,7 @@ def test_parses_good_table(tmp_path):
 
 
 @pytest.mark.parametrize("line,why", [
-    ("single fp16s N N N 10 10 10 9 1", "variant 9"),
+    ("single fp16s N N N 10 10 10 10 1", "variant 10"),
     ("grouped fp16s N N N 10 10 10 3 1", "class 3"),
     ("single fp16x N N N 10 10 10 0 1", "mode"),
     ("single fp16s N X N 10 10 10 0 1", "op"),

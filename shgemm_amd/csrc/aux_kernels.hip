@@ -13,12 +13,39 @@ namespace shgemm_amd {
 // ws element e is C element (e % inner, e / inner) in C's own storage order
 // (inner = m for column-major C, n for transposed C).
 // ---------------------------------------------------------------------------
+// Sum of p[0], p[stride], ..., p[(slices-1)*stride], added one by one in
+// index order.  The loads go out 16 at a time, the next 16 before the
+// previous ones are added: with few elements and many slices (128 x 128 C
+// over 128 slices) each thread's chain of loads is what bounds the pass.
+__device__ __forceinline__ float sum_slices(const float* __restrict__ p, int64_t stride, int slices) {
+  constexpr int D = 16;
+  const int full = slices / D * D;
+  float s = 0.0f;
+  if (full > 0) {
+    float v[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = p[(int64_t)j * stride];
+    for (int i = D; i < full; i += D) {
+      float w[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) w[j] = p[(int64_t)(i + j) * stride];
+#pragma unroll
+      for (int j = 0; j < D; ++j) s += v[j];
+#pragma unroll
+      for (int j = 0; j < D; ++j) v[j] = w[j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += v[j];
+  }
+  for (int i = full; i < slices; ++i) s += p[(int64_t)i * stride];
+  return s;
+}
+
 __device__ __forceinline__ void splitk_reduce_elem(const float* __restrict__ ws, int slices, int64_t elems, int inner,
                                                    float alpha, float beta, float* __restrict__ c, int64_t ldc,
                                                    int64_t e) {
   if (e >= elems) return;
-  float s = 0.0f;
-  for (int i = 0; i < slices; ++i) s += ws[(int64_t)i * elems + e];
+  const float s = sum_slices(ws + e, elems, slices);
   const int64_t in = e % inner, out = e / inner;
   float* cp = c + in + out * ldc;
   *cp = (beta == 0.0f) ? alpha * s : alpha * s + beta * *cp;

@@ -81,11 +81,11 @@ struct Plan {
   bool presplit = false;  // A split once per call (plan variant 8)
 };
 
-// Largest useful split: every slice at least 8 k-tiles, all partials inside
+// Largest useful split: every slice at least 4 k-tiles, all partials inside
 // the workspace.
 int64_t max_slices(const shgemm_amd::KernelEntry& e, int64_t m, int64_t n, int64_t k, size_t ws_bytes) {
   int64_t s = 1;
-  while (k / (s * 2) >= 8 * e.bk && (s * 2) * m * n * (int64_t)sizeof(float) <= (int64_t)ws_bytes && s < 256) s *= 2;
+  while (k / (s * 2) >= 4 * e.bk && (s * 2) * m * n * (int64_t)sizeof(float) <= (int64_t)ws_bytes && s < 256) s *= 2;
   return s;
 }
 

@@ -110,6 +110,25 @@ __device__ __forceinline__ int lds_slot_rm(int r, int c) {
   return r * KC + (c ^ ((r / (16 / KC)) & (KC - 1)));
 }
 
+// The same layout for the FP16 operand tiles of the SHGEMM kernels, except
+// that with 8 chunks per row (64-deep k-tiles) the key is r ^ (r >> 3): then a
+// wavefront storing one chunk of 64 consecutive rows (m-contiguous A) is
+// conflict-free too.  Measured on gfx950 with 16-byte accesses: that store
+// costs 8 conflict cycles per instruction under the window key and 0 under
+// this one; whole-row stores and the MFMA operand reads stay at 0 under both.
+// (With 4 chunks per row no key tried clears that store without costing the
+// operand reads, so 4 keeps the window key.)
+template <int KC>
+__device__ __forceinline__ int lds_key_h(int r) {
+  static_assert(KC >= 2 && KC <= 16 && (KC & (KC - 1)) == 0, "2..16 chunks per row");
+  if constexpr (KC == 8) return (r ^ (r >> 3)) & 7;
+  else return (r / (16 / KC)) & (KC - 1);
+}
+template <int KC>
+__device__ __forceinline__ int lds_slot_h(int r, int c) {
+  return r * KC + (c ^ lds_key_h<KC>(r));
+}
+
 // FP32 -> FP16 hi + 2^11-scaled FP16 lo, both round-to-nearest-even.
 // a - float(hi) is exact in FP32 (hi is the FP16 rounding of a).
 __device__ __forceinline__ void split_f16(float a, _Float16& hi, _Float16& lo) {

@@ -27,7 +27,7 @@ namespace shgemm_amd {
 // Operand tiles: R rows x BK k-values, NT threads, k cut into KC = BK/8
 // chunks of 8 values (16 bytes once in FP16/BF16).  One item = (row r,
 // chunk c): the thread loads those 8 values, converts them and writes one
-// 16-byte LDS slot per part (lds_slot_rm).
+// 16-byte LDS slot per part (lds_slot_h).
 // Element (r, k) sits at base[r*ld + k] when KCONT (k contiguous) and at
 // base[r + k*ld] otherwise.  The item -> thread maps are chosen for both
 // coalesced global loads and conflict-free LDS stores:
@@ -171,7 +171,7 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
     for (int p = 0; p < PER; ++p) {
       int r, c;
       M::item(tid + NT * p, r, c);
-      const int s = lds_slot_rm<KC>(r, c);
+      const int s = lds_slot_h<KC>(r, c);
       if constexpr (SCALED) {
         uint4 h, l;
         split2_f16_scaled(v[p][0], v[p][1], scale[p], h.x, l.x);
@@ -211,7 +211,7 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
         x1[e] = a1;
         x2[e] = a2;
       }
-      const int s = lds_slot_rm<KC>(r, c);
+      const int s = lds_slot_h<KC>(r, c);
       *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
       *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
       *reinterpret_cast<bf16x8*>(&d2[s]) = x2;
@@ -327,10 +327,10 @@ struct TileF16 {
       int r, c;
       item(tid + NT * p, r, c);
       if constexpr (KCONT) {
-        *reinterpret_cast<u32x4*>(&dst[lds_slot_rm<KC>(r, c)]) = vk[p];
+        *reinterpret_cast<u32x4*>(&dst[lds_slot_h<KC>(r, c)]) = vk[p];
       } else {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) *reinterpret_cast<u32x4*>(&dst[lds_slot_rm<KC>(r + h, c)]) = gather_row(p, h);
+        for (int h = 0; h < 2; ++h) *reinterpret_cast<u32x4*>(&dst[lds_slot_h<KC>(r + h, c)]) = gather_row(p, h);
       }
     }
   }
@@ -356,7 +356,7 @@ struct TileF16 {
         bf16x8 x0, x1;
         if constexpr (KCONT) to_bf16x2(vk[p], x0, x1);
         else to_bf16x2(gather_row(p, h), x0, x1);
-        const int s = lds_slot_rm<KC>(r + h, c);
+        const int s = lds_slot_h<KC>(r + h, c);
         *reinterpret_cast<bf16x8*>(&d0[s]) = x0;
         *reinterpret_cast<bf16x8*>(&d1[s]) = x1;
       }
@@ -369,7 +369,7 @@ struct TileF16 {
 // lane l of an instruction writes 16 bytes at the instruction's LDS base +
 // 16*l, measured on gfx950).  Instruction q of wavefront w fills physical
 // slots [(q*WAVES + w)*64, +64); the lane at physical slot (r, cp) fetches
-// logical chunk cp ^ key(r), the inverse of lds_slot_rm, so the LDS image
+// logical chunk cp ^ key(r), the inverse of lds_slot_h, so the LDS image
 // equals what TileF16::store_raw writes.  Rows past rlim are clamped (their
 // values only reach columns that are never stored); the caller guarantees a
 // full k-tile, 16-byte-aligned rows and ld % 8 == 0.
@@ -390,7 +390,7 @@ struct DirectF16 {
       const int blk = q * WAVES + wave;
       const int slot = blk * 64 + lane;
       const int r = slot / KC, cp = slot % KC;
-      const int c = cp ^ ((r / (16 / KC)) & (KC - 1));
+      const int c = cp ^ lds_key_h<KC>(r);
       const uint32_t rr = (uint32_t)(min(r0 + r, rlim - 1) - r0);
       const uint32_t boff = 2u * (rr * (uint32_t)ld + 8u * (uint32_t)c);
       __builtin_amdgcn_global_load_lds(tile + boff, reinterpret_cast<uint32_t*>(dst + blk * 64), 16, 0, 0);
@@ -516,13 +516,13 @@ __device__ __forceinline__ void mma_kstep(f32x16 (&acc)[NACC][FM][FN], const uin
       f16x8 ah[FM], al[FM], bb[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
+        const int sl = lds_slot_h<KC>(wm * WM + i * 32 + lrow, c);
         ah[i] = *reinterpret_cast<const f16x8*>(&ast[sl]);
         al[i] = *reinterpret_cast<const f16x8*>(&ast[A_CH + sl]);
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
+        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_h<KC>(wn * WN + j * 32 + lrow, c)]);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -539,14 +539,14 @@ __device__ __forceinline__ void mma_kstep(f32x16 (&acc)[NACC][FM][FN], const uin
       bf16x8 a0[FM], a1[FM], a2[FM], b0[FN], b1[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
+        const int sl = lds_slot_h<KC>(wm * WM + i * 32 + lrow, c);
         a0[i] = *reinterpret_cast<const bf16x8*>(&ast[sl]);
         a1[i] = *reinterpret_cast<const bf16x8*>(&ast[A_CH + sl]);
         a2[i] = *reinterpret_cast<const bf16x8*>(&ast[2 * A_CH + sl]);
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const int sl = lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c);
+        const int sl = lds_slot_h<KC>(wn * WN + j * 32 + lrow, c);
         b0[j] = *reinterpret_cast<const bf16x8*>(&bst[sl]);
         b1[j] = *reinterpret_cast<const bf16x8*>(&bst[B_CH + sl]);
       }
@@ -589,12 +589,12 @@ __device__ __forceinline__ void frag_load(FragF16<FM, FN>& f, const uint4* __res
   const int c = 2 * s + lh;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
-    const int sl = lds_slot_rm<KC>(wm * WM + i * 32 + lrow, c);
+    const int sl = lds_slot_h<KC>(wm * WM + i * 32 + lrow, c);
     f.ah[i] = *reinterpret_cast<const f16x8*>(&ast[sl]);
     f.al[i] = *reinterpret_cast<const f16x8*>(&ast[A_CH + sl]);
   }
 #pragma unroll
-  for (int j = 0; j < FN; ++j) f.bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
+  for (int j = 0; j < FN; ++j) f.bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_h<KC>(wn * WN + j * 32 + lrow, c)]);
 }
 // All hi products of the k-step, then all lo products: each accumulator
 // still takes hi before lo (the same bits), and FM*FN independent MFMAs
@@ -1253,7 +1253,7 @@ __device__ __forceinline__ void shgemm_tile_ring(const GemmParams& p, int m0, in
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_rm<KC>(wn * WN + j * 32 + lrow, c)]);
+        bb[j] = *reinterpret_cast<const f16x8*>(&bst[lds_slot_h<KC>(wn * WN + j * 32 + lrow, c)]);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll

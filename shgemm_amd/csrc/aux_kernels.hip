@@ -412,7 +412,7 @@ hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int ns
 // written once per call in the GEMM's tile order.  A is cut into 256-row
 // blocks and 32-deep k-tiles; tile (bm, kt) is 32 KiB at
 // out + (bm * (k / 32) + kt) * 2048 slots of 16 bytes: 1024 hi slots then
-// 1024 lo slots, slot lds_slot_rm<4>(r, c) holding row r's k-values
+// 1024 lo slots, slot lds_slot_h<4>(r, c) holding row r's k-values
 // 8c..8c+7 (the LDS image of the tile), rows past m zero.  A GEMM workgroup
 // then streams one contiguous 32 KiB block per k-tile.
 // Caller: k % 32 == 0.  One workgroup of 256 threads per tile.
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
 #pragma unroll
   for (int q = 0; q < kPackSlots / 256; ++q) {
     const int s = tid + 256 * q;
-    const int r = s / 4, c = (s % 4) ^ ((r / 4) & 3);  // inverse of lds_slot_rm<4>
+    const int r = s / 4, c = (s % 4) ^ lds_key_h<4>(r);  // inverse of lds_slot_h<4>
     float v[8];
     if constexpr (KCONT) {
       const bool in = r0 + r < m;
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
 
 // B of the pre-split plan in the same tile order: per 256-column block of n
 // and 32-deep k-tile, 16 KiB holding the tile's LDS image (slot
-// lds_slot_rm<4>(j, c) = column j's k-values 8c..8c+7, FP16 bits unchanged),
+// lds_slot_h<4>(j, c) = column j's k-values 8c..8c+7, FP16 bits unchanged),
 // columns past n zero.  B is k x n with op_b applied: op_n stores element
 // (kk, j) at b[kk + j * ldb] (k contiguous), op_t at b[j + kk * ldb].
 template <bool KCONT>
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict_
 #pragma unroll
   for (int q = 0; q < kPackSlots / 256; ++q) {
     const int s = tid + 256 * q;
-    const int j = s / 4, c = (s % 4) ^ ((j / 4) & 3);  // inverse of lds_slot_rm<4>
+    const int j = s / 4, c = (s % 4) ^ lds_key_h<4>(j);  // inverse of lds_slot_h<4>
     uint4 w;
     if constexpr (KCONT) {
       const uint16_t* src = b + (int64_t)(j0 + j) * ldb + k0 + 8 * c;

@@ -111,17 +111,17 @@ __device__ __forceinline__ int lds_slot_rm(int r, int c) {
 }
 
 // The same layout for the FP16 operand tiles of the SHGEMM kernels, except
-// that with 8 chunks per row (64-deep k-tiles) the key is r ^ (r >> 3): then a
-// wavefront storing one chunk of 64 consecutive rows (m-contiguous A) is
-// conflict-free too.  Measured on gfx950 with 16-byte accesses: that store
-// costs 8 conflict cycles per instruction under the window key and 0 under
-// this one; whole-row stores and the MFMA operand reads stay at 0 under both.
-// (With 4 chunks per row no key tried clears that store without costing the
-// operand reads, so 4 keeps the window key.)
+// that with 4 or 8 chunks per row (32- and 64-deep k-tiles) the key is
+// r ^ (r >> 2) or r ^ (r >> 3): then a wavefront storing one chunk of 64
+// consecutive rows (m-contiguous A) is conflict-free too.  Measured on gfx950
+// with 16-byte accesses: that store costs 8 conflict cycles per instruction
+// under the window key and 0 under these; whole-row stores and the MFMA
+// operand reads stay at 0 under both.
 template <int KC>
 __device__ __forceinline__ int lds_key_h(int r) {
   static_assert(KC >= 2 && KC <= 16 && (KC & (KC - 1)) == 0, "2..16 chunks per row");
   if constexpr (KC == 8) return (r ^ (r >> 3)) & 7;
+  else if constexpr (KC == 4) return (r ^ (r >> 2)) & 3;
   else return (r / (16 / KC)) & (KC - 1);
 }
 template <int KC>

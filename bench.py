@@ -521,60 +521,6 @@ def bench_bands(args):
     sg.destroy(h)
 
 
-def _energy_counter():
-    """Card readers from the ROCm SMI library: returns (joules(), sample()) --
-    the accumulated energy in joules (accumulator x resolution, in uJ) and
-    one (GFX clock MHz, socket power W) sample from the GPU metrics table."""
-    try:
-        import amdsmi
-    except ImportError as e:  # the energy workload needs the SMI library
-        raise SystemExit(f"energy workload: amdsmi is not importable ({e})")
-    amdsmi.amdsmi_init()
-    dev = amdsmi.amdsmi_get_processor_handles()[0]
-
-    def joules():
-        r = amdsmi.amdsmi_get_energy_count(dev)
-        return r["energy_accumulator"] * r["counter_resolution"] * 1e-6
-
-    def sample():
-        m = amdsmi.amdsmi_get_gpu_metrics_info(dev)
-        return m.get("current_gfxclk"), m.get("current_socket_power")
-
-    return joules, sample
-
-
-class _Sampler:
-    """Samples (clock, power) every 50 ms on a thread while active."""
-
-    def __init__(self, sample):
-        self.sample, self.out = sample, []
-
-    def __enter__(self):
-        import threading
-
-        self.stop = threading.Event()
-
-        def run():
-            while not self.stop.is_set():
-                self.out.append(self.sample())
-                time.sleep(0.05)
-
-        self.th = threading.Thread(target=run, daemon=True)
-        self.th.start()
-        return self
-
-    def __exit__(self, *exc):
-        self.stop.set()
-        self.th.join()
-
-    def medians(self):
-        import statistics
-
-        tail = self.out[len(self.out) // 3:]  # past the ramp
-        med = lambda i: statistics.median(v[i] for v in tail if isinstance(v[i], (int, float))) if tail else None
-        return med(0), med(1)
-
-
 def bench_energy(args):
     """Power and energy per GEMM (the reference's power study,
     test/power_consumption_test.cu:55-126, 230-301): m = n = k = 2^10..2^14,
@@ -589,7 +535,12 @@ def bench_energy(args):
 
     import shgemm_amd as sg
 
-    joules, sample = _energy_counter()
+    from shgemm_amd.energy import EnergyMeter, measure
+
+    try:
+        meter = EnergyMeter()
+    except RuntimeError as e:  # the energy workload needs the SMI library
+        raise SystemExit(f"energy workload: {e}")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     h = sg.create()
@@ -602,7 +553,6 @@ def bench_energy(args):
     B32 = B.float()
     A16 = A.half()
     C = torch.empty(top * top, device=dev)
-    window = args.seconds
     for log_n in range(10, 15):
         m = n = k = 1 << log_n
         calls = [("tf32", lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.tf32)),
@@ -610,28 +560,16 @@ def bench_energy(args):
                  ("sgemm_fp32", lambda: sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B32, k, 0.0, C, m)),
                  ("hipblaslt_hgemm_fp16", lambda: torch.mm(A16[:m * k].view(m, k), B[:k * n].view(k, n)))]
         for name, fn in calls:
-            fn()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(16):
-                fn()
-            torch.cuda.synchronize()
-            per_call = (time.perf_counter() - t0) / 16
-            count = max(1, int(window / per_call))
-            with _Sampler(sample) as smp:
-                e0, t0 = joules(), time.perf_counter()
-                for _ in range(count):
-                    fn()
-                torch.cuda.synchronize()
-                e1, t1 = joules(), time.perf_counter()
-            clk, pw = smp.medians()
+            r = measure(meter, fn, args.seconds, torch.cuda.synchronize)
             flop = 2.0 * m * n * k
             print(json.dumps({"compute_type": name, "m": m, "n": n, "k": k, "op_a": "N", "op_b": "N",
-                              "throughput_tflops": round(flop / per_call / 1e12, 2),
-                              "avg_power_w": round((e1 - e0) / (t1 - t0), 1),
-                              "energy_per_gemm_j": (e1 - e0) / count, "test_count": count,
-                              "gflop_per_j": round(flop * count / (e1 - e0) / 1e9, 1),
-                              "median_gfxclk_mhz": clk, "median_socket_power_w": pw}), flush=True)
+                              "throughput_tflops": round(flop / r["per_call_s"] / 1e12, 2),
+                              "avg_power_w": round(r["avg_power_w"], 1),
+                              "energy_per_gemm_j": r["energy_per_call_j"], "test_count": r["count"],
+                              "gflop_per_j": round(flop * r["count"] / r["joules"] / 1e9, 1),
+                              "median_gfxclk_mhz": r["median_gfxclk_mhz"],
+                              "median_socket_power_w": r["median_socket_power_w"]}), flush=True)
+    meter.close()
     sg.destroy(h)
 
 

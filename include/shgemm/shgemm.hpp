@@ -194,6 +194,19 @@ void enable_kernel_timing(shgemmHandle_t& handle, const int max_launches);
 // Waits for the bracketed launches since the last query and returns their
 // count and summed kernel time; the event pairs are then free again.
 void kernel_timing(shgemmHandle_t& handle, int& launches, float& total_ms);
+// Scratch and concurrency (no counterpart in the reference, whose handle
+// holds no scratch in use).  Copies of a handle share its scratch (split-K
+// partials, row exponents, the pre-split buffer); calls that use it are
+// ordered on the device after the previous such call, whatever stream each
+// runs on, and host threads may call through copies concurrently.  Growing
+// the scratch synchronises and allocates, so it never happens while the
+// handle's stream captures a graph: there the pre-split plan runs as the
+// 256x256 tile when its buffer is too small (bit-identical), and a call whose
+// row scratch would have to grow throws.  reserve() grows the scratch for
+// calls up to m x n x k beforehand.  Graphs captured with a handle must not be
+// replayed while other calls through that handle run.
+void reserve(shgemmHandle_t& handle, const std::size_t m, const std::size_t n, const std::size_t k);
+
 // The plan shgemm would use for these arguments (nothing is launched).
 plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
                   const std::size_t m, const std::size_t n, const std::size_t k, const tc_t compute_type,

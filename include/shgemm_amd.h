@@ -106,6 +106,10 @@ int shgemm_amd_force_plan(shgemm_amd_handle* h, int variant, int slices);
  * and the summed milliseconds since the last query. */
 int shgemm_amd_enable_kernel_timing(shgemm_amd_handle* h, int max_launches);
 int shgemm_amd_kernel_timing(shgemm_amd_handle* h, int* launches, float* total_ms);
+/* Grow h's scratch (row exponents, pre-split buffer) for calls up to
+ * m x n x k, so that such calls allocate nothing, e.g. while h's stream
+ * captures a graph (no counterpart in the reference). */
+int shgemm_amd_reserve(shgemm_amd_handle* h, size_t m, size_t n, size_t k);
 /* Replace h's plan table with the file at path (NULL or "": no table). */
 int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path);
 /* The plan shgemm would run for these arguments, for aligned operands. */

@@ -82,6 +82,7 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_enable_kernel_timing.argtypes = [vp, i32]
     L.shgemm_amd_kernel_timing.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_float)]
     L.shgemm_amd_force_plan.argtypes = [vp, i32, i32]
+    L.shgemm_amd_reserve.argtypes = [vp, sz, sz, sz]
     L.shgemm_amd_load_plans.argtypes = [vp, ctypes.c_char_p]
     L.shgemm_amd_query_plan.argtypes = [vp, i32, i32, sz, sz, sz, i32, i32, ip, ip, ip]
     L.shgemm_amd_check_plan_file.argtypes = [ctypes.c_char_p, ip]
@@ -245,6 +246,12 @@ def kernel_timing(handle) -> tuple[int, float]:
     ms = ctypes.c_float(0.0)
     _check(load_library().shgemm_amd_kernel_timing(handle.ptr, ctypes.byref(n), ctypes.byref(ms)))
     return n.value, float(ms.value)
+
+
+def reserve(handle, m: int, n: int, k: int) -> None:
+    """Grow the handle's scratch for calls up to m x n x k (so such calls
+    allocate nothing, e.g. under graph capture)."""
+    _check(load_library().shgemm_amd_reserve(handle.ptr, m, n, k))
 
 
 def load_plans(handle, path) -> None:

@@ -189,6 +189,13 @@ int shgemm_amd_kernel_timing(shgemm_amd_handle* h, int* launches, float* total_m
   });
 }
 
+int shgemm_amd_reserve(shgemm_amd_handle* h, size_t m, size_t n, size_t k) {
+  return guarded([&] {
+    need(h);
+    mtk::shgemm::reserve(h->h, m, n, k);
+  });
+}
+
 int shgemm_amd_load_plans(shgemm_amd_handle* h, const char* path) {
   return guarded([&] {
     need(h);

@@ -22,11 +22,10 @@ namespace shgemm {
 
 namespace {
 
-// Split-K workspace: 64 MiB of FP32 partials (reference: 3.54 MB,
-// include/shgemm/shgemm.hpp:90, sized for one 108-SM GPU).
-constexpr size_t kWorkspaceBytes = size_t(64) << 20;
-
+// Split-K workspace: shgemm_amd::kWorkspaceBytes (64 MiB) of FP32 partials
+// (reference: 3.54 MB, include/shgemm/shgemm.hpp:90, sized for one 108-SM GPU).
 using shgemm_amd::aligned;
+using shgemm_amd::kWorkspaceBytes;
 using shgemm_amd::group_rows;
 using shgemm_amd::set_lds_limit;
 
@@ -112,12 +111,17 @@ double plan_cost(const shgemm_amd::KernelEntry& e, int variant, int mode, int64_
 }
 
 // The deep-prefetch kernel applies to the prescaled fp16 mode with op_b = N,
-// 16-byte-aligned B and whole k-tiles.
+// 16-byte-aligned B and whole k-tiles, where its per-lane byte offsets fit 32
+// bits: B rows (bn of them, ldb apart, FP16), A rows of a k-contiguous op_t
+// tile (bm of them, lda apart, FP32) and, for op_n A, the row offset within a
+// column (4 * (lda - 1) at most).
 const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode, int oa, int ob, int oc, int64_t k,
-                                        bool vec_a, bool vec_b) {
+                                        bool vec_a, bool vec_b, int64_t lda, int64_t ldb) {
   const auto& bd = st->table.shgemm_bd[oa][oc];
+  constexpr int64_t kLim = int64_t(1) << 32;
+  const bool a_fits = oa == shgemm_amd::OP_N ? 4 * lda < kLim : (int64_t)bd.bm * lda * 4 < kLim;
   if (bd.fn != nullptr && mode == shgemm_amd::MODE_FP16S && ob == shgemm_amd::OP_N && vec_b && k % bd.bk == 0 &&
-      (oa == shgemm_amd::OP_N || vec_a))
+      (oa == shgemm_amd::OP_N || vec_a) && (int64_t)bd.bn * ldb * 2 < kLim && a_fits)
     return &bd;
   return nullptr;
 }
@@ -165,7 +169,7 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
     }
   } else if (requested == plan_variant_bd) {
     pl.variant = shgemm_amd::V_P1;
-    pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
+    pl.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b, lda, ldb);
     if (pl.entry != nullptr) pl.note = " deep-prefetch";
   } else if (requested == plan_variant_p0w) {
     pl.variant = shgemm_amd::V_P0;
@@ -252,7 +256,7 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
   if (best.slices == 1 && best.variant == shgemm_amd::V_P1) {
     if (apply_tile_sweep(st, best, mode, oa, ob, oc)) {
     } else if (st->use_bd) {
-      best.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b);
+      best.entry = bd_entry(st, mode, oa, ob, oc, k, vec_a, vec_b, lda, ldb);
       if (best.entry != nullptr) {
         best.requested = plan_variant_bd;
         best.note = " deep-prefetch";
@@ -260,6 +264,66 @@ Plan choose_plan(const shgemmHandle_t& handle, const detail::handle_state* st,
     }
   }
   return best;
+}
+
+// Whether `s` is capturing a graph.  A query that fails (the legacy stream
+// while another stream captures) counts as capturing: nothing may then
+// synchronise or allocate.
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// Waits on the host until every earlier call that used the handle's scratch
+// has finished (scratch_ev chains them), so that a scratch buffer may be freed.
+void drain_scratch(detail::handle_state* st) {
+  if (st->scratch_used) check_hip(hipEventSynchronize(st->scratch_ev), "hipEventSynchronize(scratch order)");
+}
+
+// Row-exponent scratch for m rows and the row pass's partial maxima.  Growing
+// it synchronises and allocates, so it cannot happen during graph capture.
+void ensure_rows(detail::handle_state* st, size_t m, bool capturing) {
+  if (st->row_cap >= m && st->row_part != nullptr) return;
+  if (capturing)
+    throw error("this call needs the handle's row scratch to grow, which cannot happen while its stream captures "
+                "a graph: call mtk::shgemm::reserve with the largest m first");
+  drain_scratch(st);
+  if (st->row_part == nullptr) {
+    check_hip(hipMalloc(&st->row_part, shgemm_amd::kRowPartialBytes), "hipMalloc(row scratch)");
+    st->row_part_bytes = shgemm_amd::kRowPartialBytes;
+  }
+  if (st->row_cap < m) {
+    if (st->row_max) (void)hipFree(st->row_max);
+    if (st->row_exp) (void)hipFree(st->row_exp);
+    st->row_max = nullptr;
+    st->row_exp = nullptr;
+    st->row_cap = 0;
+    check_hip(hipMalloc(&st->row_max, m * sizeof(uint32_t)), "hipMalloc(row scratch)");
+    check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
+    st->row_cap = m;
+  }
+}
+
+// Pre-split buffer of at least `bytes`.  False when it would have to grow
+// during graph capture or the device cannot hold it.
+bool ensure_presplit(detail::handle_state* st, size_t bytes, bool capturing) {
+  if (st->ps_bytes >= bytes) return true;
+  if (capturing) return false;
+  drain_scratch(st);
+  if (st->ps_buf) (void)hipFree(st->ps_buf);
+  st->ps_buf = nullptr;
+  st->ps_bytes = 0;
+  if (hipMalloc(&st->ps_buf, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    st->ps_buf = nullptr;
+    return false;
+  }
+  st->ps_bytes = bytes;
+  return true;
 }
 
 const char* source_name(plan_source s) {
@@ -311,10 +375,18 @@ void create(shgemmHandle_t& handle) {
           for (auto& d : c)
             for (auto& e : d)
               if (e.fn) set_lds_limit(e);
+    for (int i = 0; i < st->table.ntune; ++i) set_lds_limit(st->table.tune[i]);
+    for (auto& a : st->table.shgemm_bd)
+      for (auto& e : a)
+        if (e.fn) set_lds_limit(e);
+    for (auto& e : st->table.shgemm_ps)
+      if (e.fn) set_lds_limit(e);
+    check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
     st->ws_bytes = kWorkspaceBytes;
   } catch (...) {
     if (st->ws) (void)hipFree(st->ws);
+    if (st->scratch_ev) (void)hipEventDestroy(st->scratch_ev);
     delete st;
     throw;
   }
@@ -323,12 +395,6 @@ void create(shgemmHandle_t& handle) {
     const int i = std::atoi(tp);
     if (i < st->table.ntune) st->tune_p1 = i;
   }
-  for (int i = 0; i < st->table.ntune; ++i) set_lds_limit(st->table.tune[i]);
-  for (auto& a : st->table.shgemm_bd)
-    for (auto& e : a)
-      if (e.fn) set_lds_limit(e);
-  for (auto& e : st->table.shgemm_ps)
-    if (e.fn) set_lds_limit(e);
   const char* fp = std::getenv("MTK_SHGEMM_PLAN");
   if (fp != nullptr) {
     int v = -1, sl = 1;
@@ -354,9 +420,11 @@ void destroy(shgemmHandle_t& handle) {
   // Work enqueued by this handle may still read the scratch buffers.
   (void)hipDeviceSynchronize();
   if (st->ws) (void)hipFree(st->ws);
+  if (st->row_part) (void)hipFree(st->row_part);
   if (st->row_max) (void)hipFree(st->row_max);
   if (st->row_exp) (void)hipFree(st->row_exp);
   if (st->ps_buf) (void)hipFree(st->ps_buf);
+  if (st->scratch_ev) (void)hipEventDestroy(st->scratch_ev);
   release_timing(st);
   delete st;
   handle.state = nullptr;
@@ -375,11 +443,13 @@ void set_cuda_stream(shgemmHandle_t& handle, hipStream_t const cuda_stream) { ha
 
 void set_exponent_scaling(shgemmHandle_t& handle, const bool on) {
   if (handle.state == nullptr) throw error("handle not created");
+  std::lock_guard<std::mutex> lock(handle.state->mu);
   handle.state->exp_scaling = on;
 }
 
 void set_shared_device(shgemmHandle_t& handle, const bool on) {
   if (handle.state == nullptr) throw error("handle not created");
+  std::lock_guard<std::mutex> lock(handle.state->mu);
   handle.state->shared_device = on;
 }
 
@@ -387,6 +457,7 @@ void force_plan(shgemmHandle_t& handle, const int variant, const int slices) {
   if (handle.state == nullptr) throw error("handle not created");
   if (variant >= num_plan_variants) throw error("plan variant must be below " + std::to_string(num_plan_variants));
   if (variant >= 0 && (slices < 1 || slices > 256)) throw error("slices must be in 1..256");
+  std::lock_guard<std::mutex> lock(handle.state->mu);
   handle.state->forced_variant = variant < 0 ? -1 : variant;
   handle.state->forced_slices = variant < 0 ? 1 : slices;
 }
@@ -398,6 +469,7 @@ void load_plans(shgemmHandle_t& handle, const char* path) {
     std::string err;
     if (!shgemm_amd::parse_plan_file(path, rows, err)) throw error("plan table: " + err);
   }
+  std::lock_guard<std::mutex> lock(handle.state->mu);
   handle.state->plans.swap(rows);
   handle.state->plans_path = path != nullptr ? path : "";
 }
@@ -408,7 +480,8 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
   if ((op_a != op_n && op_a != op_t) || (op_b != op_n && op_b != op_t) || (op_c != op_n && op_c != op_t))
     throw error("invalid operation_t");
   if (compute_type != fp16 && compute_type != tf32) throw error("invalid tc_t");
-  const detail::handle_state* st = handle.state;
+  detail::handle_state* st = handle.state;
+  std::lock_guard<std::mutex> lock(st->mu);
   const int mode = compute_mode(st, compute_type);
   const int oa = op_a == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
   const int ob = op_b == op_n ? shgemm_amd::OP_N : shgemm_amd::OP_T;
@@ -427,11 +500,17 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
   return out;
 }
 
-detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
-                            const std::size_t m, const std::size_t n, const std::size_t k, const float* const alpha_ptr,
-                            const float* const a_ptr, const std::size_t lda, const half* const b_ptr,
-                            const std::size_t ldb, const float* const beta_ptr, float* const c_ptr,
-                            const std::size_t ldc, const tc_t compute_type, const operation_t op_c) {
+namespace {
+
+// One shgemm call.  `fixed` null: the handle's scratch, grown on demand and
+// ordered across streams.  `fixed` set: that scratch, and nothing is
+// allocated, freed or synchronised.
+detail::kernel_level shgemm_call(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
+                                 const std::size_t m, const std::size_t n, const std::size_t k,
+                                 const float* const alpha_ptr, const float* const a_ptr, const std::size_t lda,
+                                 const half* const b_ptr, const std::size_t ldb, const float* const beta_ptr,
+                                 float* const c_ptr, const std::size_t ldc, const tc_t compute_type,
+                                 const operation_t op_c, const shgemm_amd::CallScratch* fixed) {
   shgemm_amd::validate_args(handle, op_a, op_b, op_c, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc);
   if (compute_type != fp16 && compute_type != tf32) throw error("invalid tc_t");
   detail::handle_state* st = handle.state;
@@ -469,30 +548,31 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   p.vec_a = shgemm_amd::vec_a_ok(a_ptr, lda);
   p.vec_b = shgemm_amd::vec_b_ok(b_ptr, ldb, ob);
 
+  std::lock_guard<std::mutex> lock(st->mu);
   const auto& tab = st->table.shgemm[oa][ob][oc][mode];
   Plan plan = choose_plan(handle, st, tab, mode, oa, ob, oc, (int64_t)m, (int64_t)n, (int64_t)k, p.vec_a != 0,
                           p.vec_b != 0, p.lda, p.ldb);
+  // The capture state matters only to calls that use the handle's scratch.
+  int cap_state = -1;
+  const auto capturing = [&] {
+    if (cap_state < 0) cap_state = stream_capturing(stream) ? 1 : 0;
+    return cap_state == 1;
+  };
+  if (plan.presplit && fixed != nullptr) {
+    // no buffer for the packed operands: the 256 x 256 tile (bit-identical)
+    plan.presplit = false;
+    plan.entry = nullptr;
+    plan.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
+    plan.note = " (pre-split plan as the 256x256 tile: no allocation)";
+  }
   if (plan.presplit) {
     // A's packed hi and lo tiles (4 bytes per element, m rounded up to 256)
     // and B's packed tiles (2 bytes per element, n rounded up to 256), in a
-    // handle buffer grown on demand (growth waits for earlier work on the
-    // stream); if the device cannot hold them the call runs the
-    // register-split 256 x 256 tile.
-    const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k);
-    const size_t bytes = a_bytes + shgemm_amd::pack_b_bytes(p.n, p.k);
-    if (st->ps_bytes < bytes) {
-      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      if (st->ps_buf) (void)hipFree(st->ps_buf);
-      st->ps_buf = nullptr;
-      st->ps_bytes = 0;
-      if (hipMalloc(&st->ps_buf, bytes) == hipSuccess) {
-        st->ps_bytes = bytes;
-      } else {
-        (void)hipGetLastError();
-        st->ps_buf = nullptr;
-      }
-    }
-    if (st->ps_buf == nullptr) {
+    // handle buffer grown on demand; where the device cannot hold them, or
+    // the buffer would have to grow during graph capture, the call runs the
+    // register-split 256 x 256 tile (bit-identical).
+    const size_t bytes = shgemm_amd::presplit_bytes(p.m, p.k) + shgemm_amd::pack_b_bytes(p.n, p.k);
+    if (!ensure_presplit(st, bytes, capturing())) {
       plan.presplit = false;
       plan.entry = nullptr;
       plan.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
@@ -503,27 +583,32 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
   // Prescaled fp16: the kernels on wavefront tiles of up to 64 rows (P0, P1,
   // the split-K plans) keep running row exponents themselves; the 256 x 256
   // family and the three-stage kernel read them from a pass over A.
-  if (scaled && e.needs_row_exp) {
-    if (st->row_cap < m) {
-      // Grow the prescale scratch; earlier work on this handle may still use it.
-      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      if (st->row_max) (void)hipFree(st->row_max);
-      if (st->row_exp) (void)hipFree(st->row_exp);
-      st->row_max = nullptr;
-      st->row_exp = nullptr;
-      st->row_cap = 0;
-      check_hip(hipMalloc(&st->row_max, m * sizeof(uint32_t)), "hipMalloc(row scratch)");
-      check_hip(hipMalloc(&st->row_exp, m * sizeof(int)), "hipMalloc(row scratch)");
-      st->row_cap = m;
-    }
-    // the k-block partial maxima live in the split-K workspace: this call's
-    // GEMM uses it only after the exponent pass, in stream order
-    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, st->row_max, st->row_exp,
-                                               reinterpret_cast<uint32_t*>(st->ws), st->ws_bytes, stream),
-              "row exponent pass");
-    p.row_exp = st->row_exp;
+  const bool row_pass = scaled && e.needs_row_exp;
+  shgemm_amd::CallScratch sc;
+  if (fixed != nullptr) {
+    if (row_pass && m > fixed->row_cap)
+      throw error("m = " + std::to_string(m) + " exceeds the " + std::to_string(fixed->row_cap) +
+                  " rows of row-exponent scratch the allocation-free entry point holds for this plan");
+    sc = *fixed;
+  } else {
+    if (row_pass) ensure_rows(st, m, capturing());
+    sc.ws = st->ws;
+    sc.row_part = st->row_part;
+    sc.row_max = st->row_max;
+    sc.row_exp = st->row_exp;
+    sc.row_cap = st->row_cap;
   }
+  // calls on the handle's scratch run one after another, whatever their streams
+  const bool ordered = fixed == nullptr && (row_pass || plan.presplit || plan.slices > 1) && !capturing();
+  if (ordered && st->scratch_used && st->scratch_stream != stream)
+    check_hip(hipStreamWaitEvent(stream, st->scratch_ev, 0), "hipStreamWaitEvent(scratch order)");
 
+  if (row_pass) {
+    check_hip(shgemm_amd::launch_row_exponents(a_ptr, p.lda, oa, p.m, p.k, sc.row_max, sc.row_exp, sc.row_part,
+                                               shgemm_amd::kRowPartialBytes, stream),
+              "row exponent pass");
+    p.row_exp = sc.row_exp;
+  }
   if (plan.presplit) {
     const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k);
     uint4* apack = reinterpret_cast<uint4*>(st->ps_buf);
@@ -546,19 +631,54 @@ detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a,
         "[shape=(%zu,%zu,%zu),op_a=%s,op_b=%s,compute_type=%s] tile=%dx%dx%d slices=%d kernel_level=%d%s plan=%s%s\n",
         m, n, k, op_name(op_a), op_name(op_b), compute_type == tf32 ? "TF32" : "FP16", e.bm, e.bn, e.bk, plan.slices,
         plan.level, scaled ? " prescaled" : "", source_name(plan.source), plan.note);
+  if (handle.debug_mode) std::fflush(stdout);
   if (plan.slices > 1) {
     const int64_t kslice = (((int64_t)k + plan.slices - 1) / plan.slices + e.bk - 1) / e.bk * e.bk;
     const int64_t used = ((int64_t)k + kslice - 1) / kslice;
     p.k_slice = (int)kslice;
-    p.ws = st->ws;
+    p.ws = sc.ws;
     launch_main(st, e, dim3((unsigned)tiles, (unsigned)used), p, stream);
-    check_hip(shgemm_amd::launch_splitk_reduce(st->ws, (int)used, elems, inner, alpha, beta, c_ptr, (int64_t)ldc,
+    check_hip(shgemm_amd::launch_splitk_reduce(sc.ws, (int)used, elems, inner, alpha, beta, c_ptr, (int64_t)ldc,
                                                stream),
               "split-K reduction");
   } else {
     launch_main(st, e, dim3((unsigned)tiles), p, stream);
   }
+  if (ordered) {
+    check_hip(hipEventRecord(st->scratch_ev, stream), "hipEventRecord(scratch order)");
+    st->scratch_stream = stream;
+    st->scratch_used = true;
+  }
   return static_cast<detail::kernel_level>(plan.level);
+}
+
+}  // namespace
+
+detail::kernel_level shgemm(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b,
+                            const std::size_t m, const std::size_t n, const std::size_t k, const float* const alpha_ptr,
+                            const float* const a_ptr, const std::size_t lda, const half* const b_ptr,
+                            const std::size_t ldb, const float* const beta_ptr, float* const c_ptr,
+                            const std::size_t ldc, const tc_t compute_type, const operation_t op_c) {
+  return shgemm_call(handle, op_a, op_b, m, n, k, alpha_ptr, a_ptr, lda, b_ptr, ldb, beta_ptr, c_ptr, ldc,
+                     compute_type, op_c, nullptr);
+}
+
+void reserve(shgemmHandle_t& handle, const std::size_t m, const std::size_t n, const std::size_t k) {
+  if (handle.state == nullptr) throw error("handle not created");
+  if (m > (size_t)INT_MAX || n > (size_t)INT_MAX || k > (size_t)INT_MAX)
+    throw error("m, n and k must each be below 2^31");
+  auto* st = handle.state;
+  std::lock_guard<std::mutex> lock(st->mu);
+  if (stream_capturing(handle.cuda_stream)) throw error("reserve cannot run while the handle's stream captures a graph");
+  if (m > 0) ensure_rows(st, m, false);
+  // a handle sharing the device never runs the pre-split plan
+  if (m > 0 && n > 0 && k > 0 && !st->shared_device) {
+    // the pre-split plan needs k % 64 == 0; reserve for k rounded up to that
+    const int64_t k64 = ((int64_t)k + 63) / 64 * 64;
+    if (k64 <= INT_MAX) (void)ensure_presplit(st, shgemm_amd::presplit_bytes((int)m, (int)k64) +
+                                                      shgemm_amd::pack_b_bytes((int)n, (int)k64),
+                                              false);
+  }
 }
 
 void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const operation_t op_b, const std::size_t m,
@@ -594,6 +714,7 @@ void sgemm_fp32(const shgemmHandle_t handle, const operation_t op_a, const opera
   p.ldc = (int64_t)ldc;
   p.vec_a = aligned(a_ptr, 16) && lda % 4 == 0;
   p.vec_b = aligned(b_ptr, 16) && ldb % 4 == 0;
+  std::lock_guard<std::mutex> lock(st->mu);
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);
   p.group_m = group_rows(p.tiles_m);
@@ -609,6 +730,7 @@ void enable_kernel_timing(shgemmHandle_t& handle, const int max_launches) {
   if (handle.state == nullptr) throw error("handle not created");
   if (max_launches < 0 || max_launches > 4096) throw error("max_launches must be in 0..4096");
   auto* st = handle.state;
+  std::lock_guard<std::mutex> lock(st->mu);
   if (!st->timing.empty()) (void)hipStreamSynchronize(handle.cuda_stream);  // events may still be pending
   release_timing(st);
   for (int i = 0; i < max_launches; ++i) {
@@ -625,6 +747,7 @@ void enable_kernel_timing(shgemmHandle_t& handle, const int max_launches) {
 void kernel_timing(shgemmHandle_t& handle, int& launches, float& total_ms) {
   if (handle.state == nullptr) throw error("handle not created");
   auto* st = handle.state;
+  std::lock_guard<std::mutex> lock(st->mu);
   launches = st->timing_used;
   total_ms = 0.0f;
   for (int i = 0; i < st->timing_used; ++i) {

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -16,12 +17,24 @@ namespace mtk {
 namespace shgemm {
 namespace detail {
 
+// Copies of a shgemmHandle_t share one handle_state, so calls through them may
+// come from several host threads and on several streams.  `mu` serialises the
+// host side of each call; the scratch buffers below are ordered on the device
+// by `scratch_ev`: every call that uses them records it on its stream
+// afterwards, and a call on another stream first waits for it, so the calls
+// that touch the scratch run one after another whatever their streams.
 struct handle_state {
   int device = 0;
   int num_cus = 0;
   shgemm_amd::KernelTable table;
+  std::mutex mu;
+  hipEvent_t scratch_ev = nullptr;   // recorded after the last call that used the scratch
+  hipStream_t scratch_stream = nullptr;  // stream of that call
+  bool scratch_used = false;
   float* ws = nullptr;  // split-K partials
   size_t ws_bytes = 0;
+  uint32_t* row_part = nullptr;  // k-block partial maxima of the row-exponent pass
+  size_t row_part_bytes = 0;
   uint32_t* row_max = nullptr;  // prescale scratch
   int* row_exp = nullptr;
   size_t row_cap = 0;
@@ -70,5 +83,23 @@ inline int vec_a_ok(const float* a, size_t lda) { return aligned(a, 16) && lda %
 inline int vec_b_ok(const void* b, size_t ldb, int op_b) {
   return op_b == OP_N ? (aligned(b, 16) && ldb % 8 == 0) : (aligned(b, 8) && ldb % 4 == 0);
 }
+
+// Split-K partials of one call (the handle's workspace and the split-K part of
+// the allocation-free scratch have this size, so both choose the same plans).
+constexpr size_t kWorkspaceBytes = size_t(64) << 20;
+// k-block partial maxima of the row-exponent pass (it sizes its k-blocks to fit).
+constexpr size_t kRowPartialBytes = size_t(16) << 20;
+
+// Device memory one call may use beyond its operands, when it does not come
+// from the handle (which grows its own on demand): split-K partials of
+// kWorkspaceBytes, row-pass partial maxima of kRowPartialBytes, and per-row
+// maxima and exponents for row_cap rows.
+struct CallScratch {
+  float* ws = nullptr;
+  uint32_t* row_part = nullptr;
+  uint32_t* row_max = nullptr;
+  int* row_exp = nullptr;
+  size_t row_cap = 0;
+};
 
 }  // namespace shgemm_amd

@@ -88,17 +88,13 @@ class GpuLanes:
     def reserve(self, max_m: int, max_k: int = 0, max_l: int = 0) -> None:
         """Prepare every lane for requests up to (max_m, max_k, max_l) before
         serving, so no device allocation lands inside a request's latency:
-        run a 1-column product on each handle (its first call; a handle grows
-        its per-row scratch there if the plan reads whole-row exponents --
-        growth synchronises the lane's stream), and, when max_k and max_l are
-        given, leave room for two requests' Omega and Y in the lane stream's
-        pool of the caching allocator (pools are per stream)."""
+        grow each handle's scratch (shgemm_amd.reserve: row exponents for
+        max_m rows), and, when max_k and max_l are given, leave room for two
+        requests' Omega and Y in the lane stream's pool of the caching
+        allocator (pools are per stream)."""
         torch = self.torch
-        a = torch.zeros(max_m, device=self.device)
-        b = torch.ones(1, dtype=torch.float16, device=self.device)
-        c = torch.empty(max_m, device=self.device)
         for h in self.handles:
-            sg.shgemm(h, sg.op_n, sg.op_n, max_m, 1, 1, 1.0, a, max_m, b, 1, 0.0, c, max_m, self.compute_type)
+            sg.reserve(h, max_m, max(1, max_l), max(1, max_k))
         if max_k > 0 and max_l > 0:
             for s in self.streams:
                 with torch.cuda.stream(s):

@@ -173,3 +173,92 @@ def test_server_error_vs_fp64_and_budget(torch_mod):
         assert _f16_ulps(r.omega.cpu().numpy(), want).max() <= 1
     sg.destroy(h)
     srv.close()
+
+
+def _stated_requests(torch, pools, rows, cols, n, rng, rid0=0):
+    """Requests from the serving workload's distribution: m log-uniform in
+    [1024, 32768], k log-uniform in [1024, 16384], l in {64, ..., 1024}; A is
+    a block of one of the column-major rows x cols pools (lda = rows)."""
+    from shgemm_amd.serve import SketchRequest, sketch_shapes
+
+    reqs = []
+    for i, (m, k, l) in enumerate(sketch_shapes(n, rng)):
+        p = (rid0 + i) % len(pools)
+        r0 = int(rng.integers(0, rows - m + 1)) // 4 * 4
+        c0 = int(rng.integers(0, cols - k + 1))
+        addr = pools[p].data_ptr() + 4 * (c0 * rows + r0)
+        reqs.append(SketchRequest(rid=rid0 + i, m=m, k=k, l=l, seed=5000 + rid0 + i, a=addr, lda=rows))
+    return reqs
+
+
+def test_server_stated_distribution_alone_mix_and_budget(torch_mod):
+    """Full-size requests (A blocks of four 32768 x 16384 pools): Y is
+    bit-identical alone, in a Poisson mix at 2000 req/s under the 200 GB
+    budget, and in a mix whose budget admits only about two requests at a
+    time (the budget is never exceeded); sampled rows of Y are within 2x of
+    SGEMM's FP64 error."""
+    import shgemm_amd as sg
+    from shgemm_amd.serve import GpuLanes, SketchServer, poisson_offsets, run_arrivals
+
+    torch = torch_mod
+    ROWS, COLS, NPOOL, NREQ = 32768, 16384, 4, 24
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    pools = [torch.rand(ROWS * COLS, device="cuda", generator=g) * 2 - 1 for _ in range(NPOOL)]
+
+    def draw():
+        return _stated_requests(torch, pools, ROWS, COLS, NREQ, np.random.default_rng(33))
+
+    lanes = GpuLanes(4)
+    lanes.reserve(ROWS, COLS, 1024)
+    alone_reqs = draw()
+    srv = SketchServer(hbm_budget=200e9, slots=4, lanes=lanes, keep_omega=True)
+    alone = []
+    for r in alone_reqs:
+        srv.submit(r)
+        srv.drain()
+        torch.cuda.synchronize()
+        alone.append(r.y.cpu().numpy().copy())
+    assert max(r.m for r in alone_reqs) > 8192 and max(r.k for r in alone_reqs) > 4096
+    # accuracy of sampled rows against FP64 (A rows from the pool, Omega as generated)
+    h = sg.create()
+    rng = np.random.default_rng(8)
+    for r in alone_reqs[:8]:
+        pool = next(p for p in pools if 0 <= (r.a - p.data_ptr()) // 4 < ROWS * COLS)
+        off = (r.a - pool.data_ptr()) // 4
+        r0, c0 = off % ROWS, off // ROWS
+        rows = np.sort(rng.choice(r.m, min(r.m, 16), replace=False))
+        idx = torch.from_numpy(rows + r0).cuda()
+        A_rows = pool.view(COLS, ROWS)[c0:c0 + r.k][:, idx].t().double().cpu().numpy()
+        Om = r.omega.view(r.l, r.k).t().double().cpu().numpy()
+        ref = A_rows @ Om
+        Y = alone[alone_reqs.index(r)].reshape(r.l, r.m).T[rows].astype(np.float64)
+        Ys = torch.empty(r.m * r.l, device="cuda")
+        sg.sgemm_fp32(h, 0, 0, r.m, r.l, r.k, 1.0, r.a, r.lda, r.omega.float(), r.k, 0.0, Ys, r.m)
+        Ys_rows = Ys.view(r.l, r.m).t()[torch.from_numpy(rows).cuda()].double().cpu().numpy()
+        e = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
+        es = np.linalg.norm(Ys_rows - ref) / np.linalg.norm(ref)
+        assert e <= 2.0 * es, (r.rid, r.m, r.k, r.l, e, es)
+    sg.destroy(h)
+    srv.lanes = None  # the lanes are reused below
+
+    # the same requests in a Poisson mix at 2000 req/s, 200 GB budget
+    mix = draw()
+    srv = SketchServer(hbm_budget=200e9, slots=4, lanes=lanes)
+    st = run_arrivals(srv, mix, poisson_offsets(NREQ, 2000.0, np.random.default_rng(3)))
+    torch.cuda.synchronize()
+    assert st["requests"] == NREQ and st["peak_inflight"] >= 2
+    for r, y in zip(mix, alone):
+        assert np.array_equal(r.y.cpu().numpy().view(np.uint32), y.view(np.uint32)), r.rid
+    # a budget that admits about two of the largest requests at a time
+    tight = draw()
+    budget = int(2.2 * max(r.footprint for r in tight))
+    srv = SketchServer(hbm_budget=budget, slots=4, lanes=lanes)
+    st = run_arrivals(srv, tight, [0.0] * NREQ)
+    torch.cuda.synchronize()
+    assert st["requests"] == NREQ and srv.peak_in_use <= budget
+    for r, y in zip(tight, alone):
+        assert np.array_equal(r.y.cpu().numpy().view(np.uint32), y.view(np.uint32)), r.rid
+    lanes.close()
+    del pools
+    torch.cuda.empty_cache()

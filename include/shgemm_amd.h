@@ -55,6 +55,25 @@ int shgemm_amd_shgemm(shgemm_amd_handle* h, int op_a, int op_b,
                       const float* beta, float* c, size_t ldc,
                       int compute_type, int op_c, int* level_out);
 
+/* No counterpart in the reference: shgemm that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as shgemm_amd_shgemm, except that its scratch (split-K
+ * partials and row exponents) is one static device array of 96 MiB that every
+ * GPU the library runs on holds until the process ends, instead of h's
+ * buffers: no hipMalloc, hipFree or synchronisation happens in the call.  The
+ * pre-split plan (whose buffer grows with A) runs as the 256x256 tile, which
+ * gives the same bits.  Limit: a call whose plan reads per-row exponents
+ * (plans 5-9 of the prescaled fp16 compute type) with m above 2^21 rows
+ * returns SHGEMM_AMD_EINVAL.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int shgemm_amd_shgemm_no_alloc(shgemm_amd_handle* h, int op_a, int op_b,
+                               size_t m, size_t n, size_t k,
+                               const float* alpha, const float* a, size_t lda,
+                               const uint16_t* b, size_t ldb,
+                               const float* beta, float* c, size_t ldc,
+                               int compute_type, int op_c, int* level_out);
+
 /* mtk::shgemm::hsgemm (include/shgemm/shgemm.hpp:121-155). A FP16, B FP32. */
 int shgemm_amd_hsgemm(shgemm_amd_handle* h, int op_a, int op_b,
                       size_t m, size_t n, size_t k,

@@ -76,6 +76,7 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_set_debug_mode.argtypes = [vp, u32]
     L.shgemm_amd_shgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
     L.shgemm_amd_hsgemm.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
+    L.shgemm_amd_shgemm_no_alloc.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32, i32, ip]
     L.shgemm_amd_sgemm_fp32.argtypes = [vp, i32, i32, sz, sz, sz, fp, vp, sz, vp, sz, fp, vp, sz, i32]
     L.shgemm_amd_set_exponent_scaling.argtypes = [vp, i32]
     L.shgemm_amd_set_shared_device.argtypes = [vp, i32]
@@ -205,6 +206,17 @@ def shgemm(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, com
     _check(load_library().shgemm_amd_shgemm(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha), _dptr(a), lda,
                                             _dptr(b), ldb, _fptr(beta), _dptr(c), ldc, int(compute_type), int(op_c),
                                             ctypes.byref(lvl)))
+    return kernel_level(lvl.value)
+
+
+def shgemm_no_alloc(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, compute_type,
+                    op_c=op_n) -> kernel_level:
+    """shgemm on the library's static device scratch: allocates nothing
+    (for graph capture and callers that cannot allocate); same results."""
+    lvl = ctypes.c_int(-1)
+    _check(load_library().shgemm_amd_shgemm_no_alloc(handle.ptr, int(op_a), int(op_b), m, n, k, _fptr(alpha),
+                                                     _dptr(a), lda, _dptr(b), ldb, _fptr(beta), _dptr(c), ldc,
+                                                     int(compute_type), int(op_c), ctypes.byref(lvl)))
     return kernel_level(lvl.value)
 
 

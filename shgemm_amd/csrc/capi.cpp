@@ -10,6 +10,7 @@
 #include "errors.hpp"
 #include "plans.hpp"
 #include "registry.hpp"
+#include "runtime_internal.hpp"
 
 struct shgemm_amd_handle {
   mtk::shgemm::shgemmHandle_t h;
@@ -124,6 +125,18 @@ int shgemm_amd_shgemm(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t
     const auto lvl = mtk::shgemm::shgemm(h->h, to_op(op_a), to_op(op_b), m, n, k, alpha, a, lda,
                                          reinterpret_cast<const half*>(b), ldb, beta, c, ldc, to_tc(compute_type),
                                          to_op(op_c));
+    if (level_out) *level_out = static_cast<int>(lvl);
+  });
+}
+
+int shgemm_amd_shgemm_no_alloc(shgemm_amd_handle* h, int op_a, int op_b, size_t m, size_t n, size_t k,
+                               const float* alpha, const float* a, size_t lda, const uint16_t* b, size_t ldb,
+                               const float* beta, float* c, size_t ldc, int compute_type, int op_c, int* level_out) {
+  return guarded([&] {
+    need(h);
+    const auto lvl = shgemm_amd::shgemm_no_alloc(h->h, to_op(op_a), to_op(op_b), m, n, k, alpha, a, lda,
+                                                 reinterpret_cast<const half*>(b), ldb, beta, c, ldc,
+                                                 to_tc(compute_type), to_op(op_c));
     if (level_out) *level_out = static_cast<int>(lvl);
   });
 }

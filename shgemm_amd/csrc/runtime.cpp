@@ -382,6 +382,7 @@ void create(shgemmHandle_t& handle) {
     for (auto& e : st->table.shgemm_ps)
       if (e.fn) set_lds_limit(e);
     check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
+    shgemm_amd::init_static_scratch();
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
     st->ws_bytes = kWorkspaceBytes;
   } catch (...) {
@@ -791,6 +792,17 @@ void validate_args(const mtk::shgemm::shgemmHandle_t& handle, mtk::shgemm::opera
     if (c == nullptr) throw error("C is null");
     if (k > 0 && (a == nullptr || b == nullptr)) throw error("A or B is null");
   }
+}
+
+mtk::shgemm::detail::kernel_level shgemm_no_alloc(
+    const mtk::shgemm::shgemmHandle_t handle, mtk::shgemm::operation_t op_a, mtk::shgemm::operation_t op_b,
+    std::size_t m, std::size_t n, std::size_t k, const float* alpha, const float* a, std::size_t lda, const half* b,
+    std::size_t ldb, const float* beta, float* c, std::size_t ldc, mtk::shgemm::tc_t compute_type,
+    mtk::shgemm::operation_t op_c) {
+  if (handle.state == nullptr) throw mtk::shgemm::error("handle not created (call mtk::shgemm::create first)");
+  const CallScratch sc = static_scratch();
+  return mtk::shgemm::shgemm_call(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, compute_type, op_c,
+                                  &sc);
 }
 
 void register_all(KernelTable& t) {

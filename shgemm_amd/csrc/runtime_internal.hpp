@@ -102,4 +102,24 @@ struct CallScratch {
   size_t row_cap = 0;
 };
 
+// The allocation-free entry point's scratch on the current device: a
+// file-scope device array (static_scratch.hip) whose address is looked up
+// once per device with that device current.  init_static_scratch() does the
+// lookup for the current device (create() calls it); static_scratch()
+// returns the views, looking the address up on a device's first use when no
+// handle was created there.
+void init_static_scratch();
+CallScratch static_scratch();
+size_t static_scratch_bytes();
+
+// shgemm on the static scratch: nothing is allocated, freed or synchronised.
+// The pre-split plan (whose buffer grows with A) runs as the 256x256 tile it
+// is bit-identical to; a call whose plan needs per-row exponents for more
+// rows than the scratch holds throws mtk::shgemm::error.
+mtk::shgemm::detail::kernel_level shgemm_no_alloc(
+    const mtk::shgemm::shgemmHandle_t handle, mtk::shgemm::operation_t op_a, mtk::shgemm::operation_t op_b,
+    std::size_t m, std::size_t n, std::size_t k, const float* alpha, const float* a, std::size_t lda, const half* b,
+    std::size_t ldb, const float* beta, float* c, std::size_t ldc, mtk::shgemm::tc_t compute_type,
+    mtk::shgemm::operation_t op_c);
+
 }  // namespace shgemm_amd

@@ -8,7 +8,10 @@ resident in HBM.  Reported beside it, on the same inputs and GPU:
   * ||C - C64||_F / ||C64||_F of both against an FP64 product over a sample
     of rows (numpy float64 on the host; FP64 is used only for this check);
   * roofline of the SHGEMM kernel from HIP events on its own stream;
-  * the host FP32 GEMM (CPU baseline) on a bounded sample of rows.
+  * the host FP32 GEMM (CPU baseline): the whole product when it fits the
+    time budget, else a bounded block of rows; and the 1024^3 and sketch
+    shapes timed in full;
+  * a short sketch-serving run (p50 / p99 latency at 2000 req/s offered).
 
 Prints one JSON line.  Multi-GPU runs are not part of this build.
 """
@@ -42,8 +45,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--sample-rows", type=int, default=256, help="rows of C checked against FP64")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="square: skip the short serving run")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="square: offered rate of the serving run")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="square: arrival window of the serving run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device copies (reference for FETCH_SIZE/WRITE_SIZE scaling)")
     ap.add_argument("--problems", type=int, default=512, help="grouped workload: number of problems")
@@ -88,33 +94,66 @@ def fp64_sample_error(A_rows, B, C_rows):
     return float(d / np.linalg.norm(ref))
 
 
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
 def cpu_baseline(A_np, B32_np, m, n, k, budget_s):
-    """Host FP32 GEMM (oracle/liboracle.so) on row blocks of the same problem
-    until the time budget is used; returns (TFLOP/s, threads, rows done)."""
+    """Host FP32 GEMM (oracle.sgemm_nn_packed: packed panels, 16 x 6 AVX2 FMA
+    register tiles, every host thread) on the headline problem: the whole
+    product when a first block of rows predicts it fits the budget, else row
+    blocks until the budget is used (rate = 2 * rows * n * k / t)."""
+    import numpy as np
+
     import oracle
 
     threads = oracle.max_threads()
-    rows_per_block = 32
-    done, t = 0, 0.0
-    while t < budget_s and done < m:
-        r1 = min(m, done + rows_per_block)
+    out = np.zeros(m * n, dtype=np.float32)
+    run = lambda r0, r1: oracle.sgemm_nn_packed(m, n, k, 1.0, A_np, m, B32_np, k, 0.0, None, m, r0, r1, out=out,
+                                                threads=threads)
+    rows0 = min(m, 192 * threads)
+    t0 = time.perf_counter()
+    run(0, rows0)
+    t = time.perf_counter() - t0
+    done = rows0
+    if t * m / rows0 <= budget_s:
         t0 = time.perf_counter()
-        oracle.gemm_f32_rows(0, 0, 0, m, n, k, 1.0, A_np, m, B32_np, k, 0.0, None, m, done, r1,
-                             out=_cpu_out(m, n), threads=threads)
-        t += time.perf_counter() - t0
-        done = r1
+        run(0, m)
+        t, done = time.perf_counter() - t0, m
+    else:
+        while t < budget_s and done < m:
+            r1 = min(m, done + rows0)
+            t0 = time.perf_counter()
+            run(done, r1)
+            t += time.perf_counter() - t0
+            done = r1
     return 2.0 * done * n * k / t / 1e12, threads, done, t
 
 
-_OUT = {}
-
-
-def _cpu_out(m, n):
+def cpu_configs(A_np, B32_np, N):
+    """The same host GEMM timed in full on the 1024^3 configuration and on the
+    sketch shape 16384 x 256 x 16384 (median of 3), on blocks of the headline
+    operands (the time does not depend on the values)."""
     import numpy as np
 
-    if "c" not in _OUT:
-        _OUT["c"] = np.zeros(m * n, dtype=np.float32)
-    return _OUT["c"]
+    import oracle
+
+    threads = oracle.max_threads()
+    res = {}
+    for name, m, n, k in (("cfg1_1024x1024x1024", 1024, 1024, 1024), ("sketch_16384x256x16384", N, 256, N)):
+        A = A_np[:m * k]
+        B = B32_np[:k * n]
+        out = np.zeros(m * n, dtype=np.float32)
+        fn = lambda: oracle.sgemm_nn_packed(m, n, k, 1.0, A, m, B, k, 0.0, None, m, out=out, threads=threads)
+        fn()
+        t = _median_time(fn, 3)
+        res[name] = {"ms": round(t * 1e3, 3), "tflops": round(2.0 * m * n * k / t / 1e12, 4)}
+    return res
 
 
 def grouped_problems(count, seed):
@@ -235,40 +274,87 @@ def bench_grouped(args):
     print(json.dumps(line), flush=True)
 
 
+class _ServeSetup:
+    """Device-resident request pools (four column-major 32768 x 16384 FP32
+    matrices, U[-1,1]) and the lanes, with a request maker: m logU[1024,32768],
+    k logU[1024,16384], l in {64..1024}, A a block of one pool."""
+
+    ROWS, COLS, NPOOL = 32768, 16384, 4
+
+    def __init__(self, seed, lanes_n):
+        import numpy as np
+        import torch
+
+        from shgemm_amd.serve import GpuLanes
+
+        dev = torch.device("cuda:0")
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        self.pool = [torch.rand(self.ROWS * self.COLS, device=dev, generator=g) * 2 - 1 for _ in range(self.NPOOL)]
+        self.rng = np.random.default_rng(seed)
+        self.lanes = GpuLanes(lanes_n)
+        self.lanes.reserve(self.ROWS, self.COLS, 1024)
+
+    def make(self, n, rid0):
+        from shgemm_amd.serve import SketchRequest, sketch_shapes
+
+        out = []
+        for i, (m, k, l) in enumerate(sketch_shapes(n, self.rng)):
+            p = (rid0 + i) % self.NPOOL
+            r0 = int(self.rng.integers(0, self.ROWS - m + 1)) // 4 * 4
+            c0 = int(self.rng.integers(0, self.COLS - k + 1))
+            addr = self.pool[p].data_ptr() + 4 * (c0 * self.ROWS + r0)
+            out.append(SketchRequest(rid=rid0 + i, m=m, k=k, l=l, seed=rid0 + i, a=addr, lda=self.ROWS))
+        return out
+
+    def warm(self, slots, budget):
+        import torch
+
+        from shgemm_amd.serve import SketchServer, run_arrivals
+
+        srv = SketchServer(hbm_budget=budget, slots=slots, lanes=self.lanes)
+        run_arrivals(srv, self.make(4 * slots, 10 ** 6), [0.0] * (4 * slots))
+        torch.cuda.synchronize()
+
+    def close(self):
+        self.lanes.close()
+        self.pool = []
+
+
+def serve_summary(seed=4, rate=2000.0, seconds=2.0, lanes_n=4, budget_gb=200.0):
+    """A short serving run for the headline line: Poisson arrivals at `rate`
+    for `seconds`; p50 / p99 latency (arrival to observed completion) and the
+    sustained rate."""
+    from shgemm_amd.serve import SketchServer, poisson_offsets, run_arrivals
+
+    su = _ServeSetup(seed, lanes_n)
+    su.warm(lanes_n, budget_gb * 1e9)
+    n = max(20, int(rate * seconds))
+    srv = SketchServer(hbm_budget=budget_gb * 1e9, slots=lanes_n, lanes=su.lanes)
+    st = run_arrivals(srv, su.make(n, 0), poisson_offsets(n, rate, su.rng))
+    su.close()
+    return {"workload": "sketch server, Poisson arrivals, m logU[1024,32768], k logU[1024,16384], "
+                        "l in {64..1024}, fp16, Omega from the request seed on device",
+            "offered_req_s": rate, "requests": st["requests"], "sustained_req_s": round(st["sustained_req_s"], 1),
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "p99_bound_ms": P99_BOUND_MS, "p99_within_bound": bool(st["p99_ms"] <= P99_BOUND_MS),
+            "lanes": lanes_n, "hbm_budget_gb": budget_gb}
+
+
 def bench_serve(args):
     """Sketch serving: Poisson arrivals at each offered rate; per request an
     FP32 A (m logU[1024,32768] x k logU[1024,16384]) resident in HBM (a block
     of one of four 32768 x 16384 matrices), l in {64..1024}, Omega generated on
     the device from the request seed; p50 / p99 latency (arrival to observed
     completion) and the sustained rate."""
-    import numpy as np
     import torch
 
-    from shgemm_amd.serve import GpuLanes, SketchRequest, SketchServer, poisson_offsets, run_arrivals, sketch_shapes
+    from shgemm_amd.serve import SketchServer, poisson_offsets, run_arrivals
 
-    dev = torch.device("cuda:0")
-    ROWS, COLS, NPOOL = 32768, 16384, 4
-    g = torch.Generator(device=dev)
-    g.manual_seed(args.seed)
-    pool = [torch.rand(ROWS * COLS, device=dev, generator=g) * 2 - 1 for _ in range(NPOOL)]
-    rng = np.random.default_rng(args.seed)
-
-    def make(n, rid0):
-        out = []
-        for i, (m, k, l) in enumerate(sketch_shapes(n, rng)):
-            p = (rid0 + i) % NPOOL
-            r0 = int(rng.integers(0, ROWS - m + 1)) // 4 * 4
-            c0 = int(rng.integers(0, COLS - k + 1))
-            addr = pool[p].data_ptr() + 4 * (c0 * ROWS + r0)
-            out.append(SketchRequest(rid=rid0 + i, m=m, k=k, l=l, seed=rid0 + i, a=addr, lda=ROWS))
-        return out
-
-    lanes = GpuLanes(args.lanes)
-    lanes.reserve(ROWS, COLS, 1024)
+    su = _ServeSetup(args.seed, args.lanes)
+    lanes, make, rng = su.lanes, su.make, su.rng
     # warm-up: every lane, plan lookups and the allocator
-    srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
-    run_arrivals(srv, make(4 * args.lanes, 10 ** 6), [0.0] * (4 * args.lanes))
-    torch.cuda.synchronize()
+    su.warm(args.lanes, args.budget_gb * 1e9)
     results = []
     for rate in [float(x) for x in args.rates.split(",") if x]:
         n = max(20, int(rate * args.seconds))
@@ -286,7 +372,7 @@ def bench_serve(args):
     reqs = make(n, 9 * 10 ** 5)
     srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
     sat = run_arrivals(srv, reqs, [0.0] * n)
-    lanes.close()
+    su.close()
     ok = [r for r in results if r["sustained_req_s"] >= 0.95 * r["offered_rate"]]
     head = max(ok, key=lambda r: r["offered_rate"]) if ok else results[0]
     line = {
@@ -648,7 +734,11 @@ def main():
     value = flops / (ms_per_step * 1e-3) / 1e12
     s_value = flops / (s_region_ms / max(2, args.steps // 2) * 1e-3) / 1e12
 
-    # Error vs FP64 on a sample of rows (every C row depends on its own A row only).
+    # Error vs FP64 on a sample of rows (every C row depends on its own A row
+    # only), beside the library SGEMM's and a round-to-nearest host FP32 GEMM's
+    # on the same rows.
+    import oracle
+
     rows = np.sort(np.random.default_rng(7).choice(m, size=min(args.sample_rows, m), replace=False))
     ridx = torch.from_numpy(rows).to(dev)
     A_rows = A.view(k, m).t()[ridx].cpu().numpy()            # (rows, k)
@@ -657,25 +747,47 @@ def main():
     Cs_rows = Cs.view(n, m).t()[ridx].cpu().numpy()
     err = fp64_sample_error(A_rows, B_host, C_rows)
     err_s = fp64_sample_error(A_rows, B_host, Cs_rows)
+    nr = len(rows)
+    At = np.ascontiguousarray(A_rows.T).ravel()               # column-major nr x k
+    Bt = np.ascontiguousarray(B_host.T).ravel()               # column-major k x n
+    rn = oracle.sgemm_nn_packed(nr, n, k, 1.0, At, nr, Bt, k, 0.0, None, nr).reshape(n, nr).T
+    err_rn = fp64_sample_error(A_rows, B_host, rn)
+    del At, Bt, rn
+    # the platform's FP32 GEMM (torch.mm: hipBLASLt, no TF32) on the same rows
+    torch.backends.cuda.matmul.allow_tf32 = False
+    err_plat = fp64_sample_error(A_rows, B_host, (A.view(k, m).t()[ridx] @ B32.view(n, k).t()).cpu().numpy())
 
+    serving = None
     cpu = None
+    cpu_cfg = None
     if not args.no_cpu:
         A_np = A.cpu().numpy()
         B_np = B32.cpu().numpy()
-        del A, B16, B32, C, Cs
-        torch.cuda.empty_cache()
+    del A, B16, B32, C, Cs
+    torch.cuda.empty_cache()
+    if not args.no_serve:
+        serving = serve_summary(seed=args.seed, rate=args.serve_rate, seconds=args.serve_seconds)
+    if not args.no_cpu:
         c_tf, threads, crows, ct = cpu_baseline(A_np, B_np, m, n, k, args.cpu_seconds)
+        what = "the whole" if crows == m else f"rows 0..{crows} of the"
         cpu = {"value": round(c_tf, 6), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-               "sample": f"host FP32 GEMM (oracle/oracle.c, OpenMP, {threads} threads) on rows 0..{crows} of the "
-                         f"same {m}x{n}x{k} problem, {ct:.1f} s of CPU work; rate = 2*rows*n*k/t"}
+               "sample": f"host FP32 GEMM (oracle/oracle.c sgemm_nn_packed: packed panels, 16x6 AVX2 FMA tiles, "
+                         f"OpenMP, {threads} threads) on {what} {m}x{n}x{k} problem, {ct:.1f} s; "
+                         f"rate = 2*rows*n*k/t"}
+        cpu_cfg = cpu_configs(A_np, B_np, N)
+        del A_np, B_np
 
     achieved = flops / (kern_ms * 1e-3) / 1e12
     peak_eff = PEAK_FP16_DENSE / 2.0
-    traffic = None
+    traffic, traffic_src = None, None
     tf = ROOT / "profiles" / "traffic_shgemm_16384.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            traffic = tj.get("hbm_bytes_per_launch")
+            traffic_src = (f"stored, not measured in this run: {tf.relative_to(ROOT)} "
+                           f"(rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of {tj.get('kernel', 'the GEMM kernel')}, "
+                           f"build {tj.get('build', 'unrecorded')})")
         except Exception:
             traffic = None
     line = {
@@ -697,15 +809,26 @@ def main():
                        "sgemm_kernel_ms": round(s_kern_ms, 4),
                        "sgemm_roofline_frac": round(s_value / PEAK_FP32_MATRIX, 4)},
         "error": {"shgemm_rel_fro": err, "sgemm_rel_fro": err_s, "ratio": err / err_s,
-                  "sample_rows": int(len(rows)), "oracle": "numpy float64 product of the sampled rows"},
+                  "platform_fp32_gemm_rel_fro": err_plat,
+                  "host_rn_fp32_rel_fro": err_rn, "ratio_vs_host_rn_fp32": err / err_rn,
+                  "sample_rows": int(nr),
+                  "oracle": "numpy float64 product of the sampled rows; platform_fp32_gemm = torch.mm in FP32 "
+                            "(hipBLASLt) on those rows; host_rn_fp32 = the packed host FP32 GEMM (round-to-nearest "
+                            "FMA chain over k) on the same rows"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak_eff, 1), "unit": "TFLOP/s",
-                     "frac": round(achieved / peak_eff, 4), "traffic": traffic,
-                     "note": "achieved = 2mnk / mean duration of the main GEMM kernel (HIP events the library "
-                             "records around it on the launch stream; the row-exponent pass is excluded); "
-                             "peak = FP16 dense 2516.6 / 2, since every algorithmic FMA costs two FP16 MFMA "
-                             "products (hi and lo)",
+                     "frac": round(achieved / peak_eff, 4),
+                     "frac_of_fp16_dense_peak": round(achieved / PEAK_FP16_DENSE, 4),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "note": "achieved = 2mnk / mean duration of the main GEMM kernel alone (HIP events the library "
+                             "records around it on the launch stream); left out of that duration and counted in "
+                             "ms_per_step: the row-exponent pass over A and, on the pre-split plan, the pack "
+                             "pass of A (split into FP16 hi/lo) and of B; peak = FP16 dense 2516.6 / 2, since "
+                             "every algorithmic FMA costs two FP16 MFMA products (hi and lo); "
+                             "frac_of_fp16_dense_peak = achieved / 2516.6",
                      "kernel_ms": round(kern_ms, 4)},
         "cpu_baseline": cpu,
+        "cpu_configs": cpu_cfg,
+        "sketch_serving": serving,
         "wall_ms_timed_region": round(wall_ms, 3),
     }
     print(json.dumps(line), flush=True)

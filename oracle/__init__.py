@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
                                            i64, i64, i32]
         L.oracle_gemm_f32_rows.argtypes = [i32, i32, i32, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, vp,
                                            i64, i64, i32]
+        L.oracle_sgemm_nn_packed.argtypes = [i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, vp, i64, i64, i32]
         L.oracle_split_f16.argtypes = [vp, vp, vp, i64]
         L.oracle_split_bf16x3.argtypes = [vp, vp, vp, vp, i64]
         L.oracle_split_f16_same_scale.argtypes = [vp, vp, vp, i64]
@@ -120,6 +121,23 @@ def gemm_f32_rows(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ld
         out = np.zeros(size, dtype=np.float32)
     lib().oracle_gemm_f32_rows(op_a, op_b, op_c, m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb, float(beta),
                                _ptr(C0), ldc, _ptr(out), r0, r1, threads)
+    return out
+
+
+def sgemm_nn_packed(m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, r0=0, r1=None, out=None, threads=0):
+    """Host FP32 GEMM, op_n/op_n/op_n, rows [r0, r1) of C: packed panels and a
+    16 x 6 AVX2 FMA register tile (the timed CPU baseline).  Each output is a
+    round-to-nearest FP32 FMA chain over k in order."""
+    A = _check_f32(A, "A")
+    B = _check_f32(B, "B")
+    r1 = m if r1 is None else r1
+    if out is None:
+        out = np.zeros(ldc * n, dtype=np.float32)
+    if C0 is not None and beta != 0.0:
+        C0 = _check_f32(C0, "C0")
+    lib().oracle_sgemm_nn_packed(m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb, float(beta),
+                                 _ptr(C0) if (C0 is not None and beta != 0.0) else None, ldc, _ptr(out), r0, r1,
+                                 threads)
     return out
 
 

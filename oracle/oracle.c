@@ -8,8 +8,11 @@
  *                    reference's tests do: test/shgemm_test.cu:214-224, 91-100).
  *                    Replaces mateval::get_error_AxB's reference product
  *                    (un-vendored submodule test/mateval, .gitmodules:7-9).
- *   oracle_gemm_f32  the same product with FP32 accumulation: the host SGEMM
- *                    that is timed as the CPU baseline.
+ *   oracle_gemm_f32  the same product with FP32 accumulation (any op layout).
+ *   oracle_sgemm_nn_packed  the host FP32 GEMM timed as the CPU baseline
+ *                    (op_n / op_n / op_n): packed panels and a 16 x 6 AVX2 FMA
+ *                    register tile, the standard blocking of high-performance
+ *                    CPU GEMM (Goto & van de Geijn, ACM TOMS 34(3), 2008).
  *   oracle_split_f16 hi = fp16(a), lo = fp16((a - hi) * 2^11), both
  *                    round-to-nearest-even (reference src/wmmae_shgemm.hpp:29-31;
  *                    method of Ootomo & Yokota, IJHPCA 2022).
@@ -27,6 +30,7 @@
  * (round to nearest, ties to even; overflow to infinity; gradual underflow).
  * Layout: column-major; op_t means the operand is stored transposed.
  */
+#include <immintrin.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -295,6 +299,132 @@ void oracle_gemm_f32(int op_a, int op_b, int op_c, int64_t m, int64_t n, int64_t
                      int64_t lda, const float* B, int64_t ldb, float beta, const float* C0, int64_t ldc, float* out,
                      int nthreads) {
   oracle_gemm_f32_rows(op_a, op_b, op_c, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, out, 0, m, nthreads);
+}
+
+/* ---------------------------------------------------------------- */
+/* Packed FP32 GEMM, op_n / op_n / op_n, rows [r0, r1) of C           */
+/* ---------------------------------------------------------------- */
+enum { PK_MR = 16, PK_NR = 6, PK_MC = 192, PK_KC = 256, PK_NC = 4032 };
+
+/* acc(16 x 6) = sum over p < kc of ap[16p .. 16p+15] x bp[6p .. 6p+5]; then
+ * C = alpha * acc + beta_eff * C on the mr x nr corner that exists. */
+static void pk_tile(int64_t kc, const float* ap, const float* bp, float* c, int64_t ldc, int mr, int nr, float alpha,
+                    float beta_eff) {
+  /* twelve named accumulators so that they stay in registers */
+  __m256 c00 = _mm256_setzero_ps(), c01 = c00, c02 = c00, c03 = c00, c04 = c00, c05 = c00;
+  __m256 c10 = c00, c11 = c00, c12 = c00, c13 = c00, c14 = c00, c15 = c00;
+  for (int64_t p = 0; p < kc; ++p) {
+    const __m256 a0 = _mm256_load_ps(ap + PK_MR * p);
+    const __m256 a1 = _mm256_load_ps(ap + PK_MR * p + 8);
+    const float* b = bp + PK_NR * p;
+    __m256 bj = _mm256_broadcast_ss(b + 0);
+    c00 = _mm256_fmadd_ps(a0, bj, c00);
+    c10 = _mm256_fmadd_ps(a1, bj, c10);
+    bj = _mm256_broadcast_ss(b + 1);
+    c01 = _mm256_fmadd_ps(a0, bj, c01);
+    c11 = _mm256_fmadd_ps(a1, bj, c11);
+    bj = _mm256_broadcast_ss(b + 2);
+    c02 = _mm256_fmadd_ps(a0, bj, c02);
+    c12 = _mm256_fmadd_ps(a1, bj, c12);
+    bj = _mm256_broadcast_ss(b + 3);
+    c03 = _mm256_fmadd_ps(a0, bj, c03);
+    c13 = _mm256_fmadd_ps(a1, bj, c13);
+    bj = _mm256_broadcast_ss(b + 4);
+    c04 = _mm256_fmadd_ps(a0, bj, c04);
+    c14 = _mm256_fmadd_ps(a1, bj, c14);
+    bj = _mm256_broadcast_ss(b + 5);
+    c05 = _mm256_fmadd_ps(a0, bj, c05);
+    c15 = _mm256_fmadd_ps(a1, bj, c15);
+  }
+  float t[PK_NR][PK_MR] __attribute__((aligned(32)));
+  _mm256_store_ps(t[0], c00);
+  _mm256_store_ps(t[0] + 8, c10);
+  _mm256_store_ps(t[1], c01);
+  _mm256_store_ps(t[1] + 8, c11);
+  _mm256_store_ps(t[2], c02);
+  _mm256_store_ps(t[2] + 8, c12);
+  _mm256_store_ps(t[3], c03);
+  _mm256_store_ps(t[3] + 8, c13);
+  _mm256_store_ps(t[4], c04);
+  _mm256_store_ps(t[4] + 8, c14);
+  _mm256_store_ps(t[5], c05);
+  _mm256_store_ps(t[5] + 8, c15);
+  for (int j = 0; j < nr; ++j) {
+    float* cj = c + j * ldc;
+    for (int i = 0; i < mr; ++i) cj[i] = alpha * t[j][i] + (beta_eff != 0.0f ? beta_eff * cj[i] : 0.0f);
+  }
+}
+
+/* out = alpha * A * B + beta * C0 on rows [r0, r1) (out may be C0). */
+void oracle_sgemm_nn_packed(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                            const float* B, int64_t ldb, float beta, const float* C0, int64_t ldc, float* out,
+                            int64_t r0, int64_t r1, int nthreads) {
+  (void)m;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  if (r1 <= r0 || n <= 0) return;
+  if (C0 != NULL && C0 != out && beta != 0.0f)
+    for (int64_t j = 0; j < n; ++j) memcpy(out + j * ldc + r0, C0 + j * ldc + r0, sizeof(float) * (r1 - r0));
+  if (k <= 0) {
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = r0; i < r1; ++i) out[i + j * ldc] = beta != 0.0f ? beta * out[i + j * ldc] : 0.0f;
+    return;
+  }
+  float* bpk = (float*)aligned_alloc(64, sizeof(float) * PK_KC * PK_NC);
+  for (int64_t jc = 0; jc < n; jc += PK_NC) {
+    const int64_t nc = (n - jc < PK_NC) ? n - jc : PK_NC;
+    const int64_t nsl = (nc + PK_NR - 1) / PK_NR;
+    for (int64_t pc = 0; pc < k; pc += PK_KC) {
+      const int64_t kc = (k - pc < PK_KC) ? k - pc : PK_KC;
+      const float beta_eff = pc == 0 ? beta : 1.0f;
+      /* B panel in slivers of 6 columns: bpk[s*6*kc + 6p + j] = B(pc + p, jc + 6s + j) */
+#pragma omp parallel for schedule(static)
+      for (int64_t sl = 0; sl < nsl; ++sl) {
+        float* dst = bpk + sl * PK_NR * kc;
+        for (int j = 0; j < PK_NR; ++j) {
+          const int64_t col = jc + sl * PK_NR + j;
+          if (col < n) {
+            const float* src = B + col * ldb + pc;
+            for (int64_t p = 0; p < kc; ++p) dst[PK_NR * p + j] = src[p];
+          } else {
+            for (int64_t p = 0; p < kc; ++p) dst[PK_NR * p + j] = 0.0f;
+          }
+        }
+      }
+#pragma omp parallel
+      {
+        float* apk = (float*)aligned_alloc(64, sizeof(float) * PK_MC * PK_KC);
+#pragma omp for schedule(dynamic)
+        for (int64_t ic = r0; ic < r1; ic += PK_MC) {
+          const int64_t mc = (r1 - ic < PK_MC) ? r1 - ic : PK_MC;
+          /* A block in slivers of 16 rows: apk[s*16*kc + 16p + i] = A(ic + 16s + i, pc + p) */
+          for (int64_t s0 = 0; s0 < mc; s0 += PK_MR) {
+            float* dst = apk + (s0 / PK_MR) * PK_MR * kc;
+            const int64_t rows = (mc - s0 < PK_MR) ? mc - s0 : PK_MR;
+            for (int64_t p = 0; p < kc; ++p) {
+              const float* src = A + (pc + p) * lda + ic + s0;
+              int64_t i = 0;
+              for (; i < rows; ++i) dst[PK_MR * p + i] = src[i];
+              for (; i < PK_MR; ++i) dst[PK_MR * p + i] = 0.0f;
+            }
+          }
+          for (int64_t jr = 0; jr < nc; jr += PK_NR) {
+            const int nr = (int)((nc - jr < PK_NR) ? nc - jr : PK_NR);
+            for (int64_t ir = 0; ir < mc; ir += PK_MR) {
+              const int mr = (int)((mc - ir < PK_MR) ? mc - ir : PK_MR);
+              pk_tile(kc, apk + (ir / PK_MR) * PK_MR * kc, bpk + (jr / PK_NR) * PK_NR * kc,
+                      out + (ic + ir) + (jc + jr) * ldc, ldc, mr, nr, pc == 0 ? alpha : alpha, beta_eff);
+            }
+          }
+        }
+        free(apk);
+      }
+    }
+  }
+  free(bpk);
 }
 
 int oracle_max_threads(void) {

@@ -191,6 +191,39 @@ def test_gemm_f32_is_fp32_accurate():
     assert np.array_equal(v[64:128], w[64:128])
 
 
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (15, 5, 7), (37, 29, 300), (200, 130, 513), (400, 4033, 260)])
+def test_sgemm_nn_packed_exact_on_integers(m, n, k):
+    """Integer inputs with |a|, |b| <= 8 and k <= 513: every partial sum is an
+    integer below 2^24, so the packed FP32 GEMM must equal the FP64 product
+    exactly, over ragged tiles (m % 16, n % 6, k % 256, n past one 4032-column
+    panel), alpha/beta and a row range."""
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.integers(-8, 9, m * k).astype(np.float32)
+    B = rng.integers(-8, 9, k * n).astype(np.float32)
+    C0 = rng.integers(-8, 9, (m + 3) * n).astype(np.float32)
+    ref = A.reshape(k, m).T.astype(np.float64) @ B.reshape(n, k).T.astype(np.float64)
+    got = oracle.sgemm_nn_packed(m, n, k, 1.0, A, m, B, k, 0.0, None, m)
+    assert np.array_equal(got.reshape(n, m).T.astype(np.float64), ref)
+    got = oracle.sgemm_nn_packed(m, n, k, 2.0, A, m, B, k, -1.0, C0, m + 3)
+    want = 2.0 * ref - C0.reshape(n, m + 3).T[:m].astype(np.float64)
+    assert np.array_equal(got.reshape(n, m + 3).T[:m].astype(np.float64), want)
+    r0, r1 = m // 3, m - m // 4
+    part = oracle.sgemm_nn_packed(m, n, k, 1.0, A, m, B, k, 0.0, None, m, r0, r1)
+    v = part.reshape(n, m).T
+    assert np.array_equal(v[r0:r1].astype(np.float64), ref[r0:r1]) and not v[:r0].any() and not v[r1:].any()
+
+
+def test_sgemm_nn_packed_is_fp32_accurate():
+    rng = np.random.default_rng(14)
+    m, n, k = 300, 200, 4096
+    A = oracle.gen_uniform(rng, m * k)
+    B = oracle.gen_uniform(rng, k * n)
+    f64 = oracle.gemm_f64(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m)
+    err_p = oracle.rel_fro_error(oracle.sgemm_nn_packed(m, n, k, 1.0, A, m, B, k, 0.0, None, m), f64)
+    err_f = oracle.rel_fro_error(oracle.gemm_f32(0, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, None, m), f64)
+    assert 1e-9 < err_p < 2.0 * err_f
+
+
 def test_emulation_and_planted_mistakes():
     """The 2x-SGEMM bar of the GPU tests rejects each plausible kernel mistake."""
     rng = np.random.default_rng(14)

@@ -53,10 +53,13 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device copies (reference for FETCH_SIZE/WRITE_SIZE scaling)")
     ap.add_argument("--problems", type=int, default=512, help="grouped workload: number of problems")
+    ap.add_argument("--group-only", action="store_true", help="grouped: skip the per-problem comparison launches")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--rates", default="200,500,1000,2000", help="serve: offered request rates (req/s)")
     ap.add_argument("--seconds", type=float, default=3.0, help="serve: arrival window per rate; energy: measuring window per GEMM")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent lanes (handle + stream each)")
+    ap.add_argument("--max-batch", type=int, default=1,
+                    help="serve: requests admitted together to one lane run as one grouped launch (up to this many)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget")
     return ap.parse_args()
 
@@ -202,10 +205,13 @@ def bench_grouped(args):
         for m, n, k, A, lda, B, ldb, C, ldc, al, be in probs:
             sg.shgemm(h, sg.op_n, sg.op_n, m, n, k, al, A, lda, B, ldb, be, C, ldc, sg.fp16)
 
-    p_region, _, p_ms = timed_launches(per_problem, max(2, args.steps // 2), 1, stream)
+    p_region, pg_ms = None, None
+    if not args.group_only:
+        p_region, _, p_ms = timed_launches(per_problem, max(2, args.steps // 2), 1, stream)
     # the same per-problem launches replayed from one captured graph
-    pg_ms = None
     try:
+        if args.group_only:
+            raise RuntimeError("skipped (--group-only)")
         cs = torch.cuda.Stream()
         sg.set_cuda_stream(h, cs)
         graph = torch.cuda.CUDAGraph()
@@ -241,7 +247,14 @@ def bench_grouped(args):
     ms_per_step = g_region / args.steps
     value = flops / (ms_per_step * 1e-3) / 1e12
     t_roof = max(flops / (PEAK_FP16_DENSE / 2.0 * 1e12), byts / (PEAK_HBM * 1e12))
-    hist = {c: int(sum(1 for x in cls if x == c)) for c in (0, 1, 2)}
+    hist = {c: int(sum(1 for x in cls if x == c)) for c in (0, 1, 2, 3)}
+    per_class = {}
+    for c in (0, 1, 2, 3):
+        sel = [i for i, x in enumerate(cls) if x == c]
+        per_class[c] = {"problems": len(sel),
+                        "gflop": round(sum(2.0 * shapes[i][0] * shapes[i][1] * shapes[i][2] for i in sel) / 1e9, 1),
+                        "gbytes": round(sum(4.0 * shapes[i][0] * shapes[i][2] + 2.0 * shapes[i][2] * shapes[i][1]
+                                            + 4.0 * shapes[i][0] * shapes[i][1] for i in sel) / 1e9, 3)}
     line = {
         "metric": "eff. TFLOP/s, batched uneven SHGEMM (one grouped launch)",
         "value": round(value, 3),
@@ -258,12 +271,14 @@ def bench_grouped(args):
         "config": {"workload": f"grouped shgemm, {len(shapes)} problems, m logU[256,8192], n in {{64..512}}, "
                                f"k logU[256,16384], A U[-1,1], B N(0,1/n), op_n/op_n fp16",
                    "problems": len(shapes), "total_flops": flops, "total_bytes": byts,
-                   "work_units": units, "tile_classes": hist,
+                   "work_units": units, "tile_classes": hist, "per_class": per_class,
+                   "classes_side_by_side": os.environ.get("MTK_SHGEMM_GROUP_CONCURRENT", "0") == "1",
                    "split_problems": int(sum(1 for s in slices if s > 1)), "parallelism": "single GPU"},
-        "comparator": {"per_problem_launch_ms": round(p_region / max(2, args.steps // 2), 4),
-                       "per_problem_launch_tflops": round(flops / (p_region / max(2, args.steps // 2) * 1e-3) / 1e12, 3),
-                       "per_problem_graph_ms": round(pg_ms, 4) if pg_ms else None,
-                       "grouped_speedup_vs_per_problem": round(p_region / max(2, args.steps // 2) / ms_per_step, 3)},
+        "comparator": None if p_region is None else {
+            "per_problem_launch_ms": round(p_region / max(2, args.steps // 2), 4),
+            "per_problem_launch_tflops": round(flops / (p_region / max(2, args.steps // 2) * 1e-3) / 1e12, 3),
+            "per_problem_graph_ms": round(pg_ms, 4) if pg_ms else None,
+            "grouped_speedup_vs_per_problem": round(p_region / max(2, args.steps // 2) / ms_per_step, 3)},
         "error": {"max_ratio_vs_sgemm": max(errs), "sampled_problems": len(errs), "rows_per_problem": 64},
         "roofline": {"bound": "mfma" if flops / (PEAK_FP16_DENSE / 2.0 * 1e12) > byts / (PEAK_HBM * 1e12) else "hbm",
                      "achieved": round(flops / (g_ms * 1e-3) / 1e12, 3), "peak": round(PEAK_FP16_DENSE / 2.0, 1),
@@ -281,7 +296,7 @@ class _ServeSetup:
 
     ROWS, COLS, NPOOL = 32768, 16384, 4
 
-    def __init__(self, seed, lanes_n):
+    def __init__(self, seed, lanes_n, max_batch=1):
         import numpy as np
         import torch
 
@@ -292,7 +307,8 @@ class _ServeSetup:
         g.manual_seed(seed)
         self.pool = [torch.rand(self.ROWS * self.COLS, device=dev, generator=g) * 2 - 1 for _ in range(self.NPOOL)]
         self.rng = np.random.default_rng(seed)
-        self.lanes = GpuLanes(lanes_n)
+        self.max_batch = max_batch
+        self.lanes = GpuLanes(lanes_n, max_batch=max_batch if max_batch > 1 else 0)
         self.lanes.reserve(self.ROWS, self.COLS, 1024)
 
     def make(self, n, rid0):
@@ -312,7 +328,7 @@ class _ServeSetup:
 
         from shgemm_amd.serve import SketchServer, run_arrivals
 
-        srv = SketchServer(hbm_budget=budget, slots=slots, lanes=self.lanes)
+        srv = SketchServer(hbm_budget=budget, slots=slots, lanes=self.lanes, max_batch=self.max_batch)
         run_arrivals(srv, self.make(4 * slots, 10 ** 6), [0.0] * (4 * slots))
         torch.cuda.synchronize()
 
@@ -321,16 +337,16 @@ class _ServeSetup:
         self.pool = []
 
 
-def serve_summary(seed=4, rate=2000.0, seconds=2.0, lanes_n=4, budget_gb=200.0):
+def serve_summary(seed=4, rate=2000.0, seconds=2.0, lanes_n=4, budget_gb=200.0, max_batch=1):
     """A short serving run for the headline line: Poisson arrivals at `rate`
     for `seconds`; p50 / p99 latency (arrival to observed completion) and the
     sustained rate."""
     from shgemm_amd.serve import SketchServer, poisson_offsets, run_arrivals
 
-    su = _ServeSetup(seed, lanes_n)
+    su = _ServeSetup(seed, lanes_n, max_batch)
     su.warm(lanes_n, budget_gb * 1e9)
     n = max(20, int(rate * seconds))
-    srv = SketchServer(hbm_budget=budget_gb * 1e9, slots=lanes_n, lanes=su.lanes)
+    srv = SketchServer(hbm_budget=budget_gb * 1e9, slots=lanes_n, lanes=su.lanes, max_batch=max_batch)
     st = run_arrivals(srv, su.make(n, 0), poisson_offsets(n, rate, su.rng))
     su.close()
     return {"workload": "sketch server, Poisson arrivals, m logU[1024,32768], k logU[1024,16384], "
@@ -338,7 +354,8 @@ def serve_summary(seed=4, rate=2000.0, seconds=2.0, lanes_n=4, budget_gb=200.0):
             "offered_req_s": rate, "requests": st["requests"], "sustained_req_s": round(st["sustained_req_s"], 1),
             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
             "p99_bound_ms": P99_BOUND_MS, "p99_within_bound": bool(st["p99_ms"] <= P99_BOUND_MS),
-            "lanes": lanes_n, "hbm_budget_gb": budget_gb}
+            "lanes": lanes_n, "max_batch": max_batch, "mean_batch": round(st["mean_batch"], 2),
+            "hbm_budget_gb": budget_gb}
 
 
 def bench_serve(args):
@@ -351,7 +368,7 @@ def bench_serve(args):
 
     from shgemm_amd.serve import SketchServer, poisson_offsets, run_arrivals
 
-    su = _ServeSetup(args.seed, args.lanes)
+    su = _ServeSetup(args.seed, args.lanes, args.max_batch)
     lanes, make, rng = su.lanes, su.make, su.rng
     # warm-up: every lane, plan lookups and the allocator
     su.warm(args.lanes, args.budget_gb * 1e9)
@@ -359,7 +376,7 @@ def bench_serve(args):
     for rate in [float(x) for x in args.rates.split(",") if x]:
         n = max(20, int(rate * args.seconds))
         reqs = make(n, len(results) * 10 ** 5)
-        srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
+        srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes, max_batch=args.max_batch)
         st = run_arrivals(srv, reqs, poisson_offsets(n, rate, rng))
         st["offered_rate"] = rate
         st["mean_request_gflop"] = sum(r.flops for r in reqs) / n / 1e9
@@ -370,7 +387,7 @@ def bench_serve(args):
     # saturation: every request present at time zero
     n = max(40, int(max(r["offered_rate"] for r in results) * args.seconds / 2))
     reqs = make(n, 9 * 10 ** 5)
-    srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes)
+    srv = SketchServer(hbm_budget=args.budget_gb * 1e9, slots=args.lanes, lanes=lanes, max_batch=args.max_batch)
     sat = run_arrivals(srv, reqs, [0.0] * n)
     su.close()
     ok = [r for r in results if r["sustained_req_s"] >= 0.95 * r["offered_rate"]]
@@ -390,7 +407,8 @@ def bench_serve(args):
         "data": "synthetic: A resident (seeded torch RNG), Omega from request seed on device",
         "config": {"workload": "sketch server, Poisson arrivals, m logU[1024,32768], k logU[1024,16384], "
                                "l in {64,128,256,512,1024}, fp16 compute type",
-                   "lanes": args.lanes, "hbm_budget_gb": args.budget_gb, "seconds_per_rate": args.seconds,
+                   "lanes": args.lanes, "max_batch": args.max_batch, "hbm_budget_gb": args.budget_gb,
+                   "seconds_per_rate": args.seconds,
                    "parallelism": "single GPU"},
         "sustained_req_s": round(head["sustained_req_s"], 1),
         "offered_req_s": head["offered_rate"],
@@ -401,6 +419,7 @@ def bench_serve(args):
         "p99_within_bound": all(r["p99_ms"] <= P99_BOUND_MS for r in ok) and bool(ok),
         "per_rate": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
         "saturation": {"requests": sat["requests"], "req_s": round(sat["sustained_req_s"], 1),
+                       "mean_batch": round(sat["mean_batch"], 2),
                        "tflops": round(sat["flops_per_s"] / 1e12, 2)},
         "roofline": None,
         "cpu_baseline": None,
@@ -766,7 +785,8 @@ def main():
     del A, B16, B32, C, Cs
     torch.cuda.empty_cache()
     if not args.no_serve:
-        serving = serve_summary(seed=args.seed, rate=args.serve_rate, seconds=args.serve_seconds)
+        serving = serve_summary(seed=args.seed, rate=args.serve_rate, seconds=args.serve_seconds,
+                                max_batch=args.max_batch)
     if not args.no_cpu:
         c_tf, threads, crows, ct = cpu_baseline(A_np, B_np, m, n, k, args.cpu_seconds)
         what = "the whole" if crows == m else f"rows 0..{crows} of the"

@@ -217,7 +217,8 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
 // share op_a, op_b, op_c and compute_type, run by persistent kernels that
 // take (problem, k-slice, tile) work units from a queue, longest first.  Each
 // problem gets a tile class (0 small, 1 128x128, 2 the large tile) and a
-// k-slice count from the plan table's grouped entries (or the arguments).
+// k-slice count from the plan table's grouped entries (or the arguments);
+// class 3, a 128x512 tile, exists for the prescaled fp16 mode only.
 // create_group validates every problem, allocates the group's descriptors
 // and split-K partials, and captures the whole sequence (one launch per tile
 // class used, split-K reduction; the tile classes keep running row
@@ -242,6 +243,29 @@ void run_group(group_t* group, hipStream_t stream, const bool use_graph = true);
 void destroy_group(group_t* group);
 // Per-problem choices (tile class, slices) and the group's work-unit count.
 void group_plan(const group_t* group, int* tile_class, int* slices, std::size_t* units);
+
+// ---- Batches -----------------------------------------------------------------
+// Problems that change from call to call (a serving loop's admitted
+// requests) run as one grouped launch without allocating: create_batch
+// allocates `slots` device regions of `bytes` each (descriptors and split-K
+// partials of one batch) and pinned staging for up to max_problems
+// descriptors; run_batch plans the problems exactly as create_group would
+// (tile class and k-slices from the plan table's grouped rows), copies their
+// descriptors into the next slot asynchronously and enqueues fill, the
+// persistent launches and the split-K reduction on `stream`.  A slot is
+// reused only after its previous batch has finished (run_batch waits on the
+// host for that when all slots are in flight).  Results are bit-identical to
+// the same problems in a group and to single calls with the matching plans.
+// run_batch throws if the batch needs more than `bytes` of device scratch.
+struct batch_t;
+batch_t* create_batch(const shgemmHandle_t& handle, const operation_t op_a, const operation_t op_b,
+                      const tc_t compute_type, const operation_t op_c, const std::size_t max_problems,
+                      const std::size_t bytes, const int slots = 2);
+void run_batch(batch_t* batch, const group_problem* problems, const std::size_t count, hipStream_t stream);
+// Tile class and slices of every problem of the last run_batch, and its work units.
+void batch_plan(const batch_t* batch, int* tile_class, int* slices, std::size_t* units);
+// Waits for the batch's launches in flight, then frees it.
+void destroy_batch(batch_t* batch);
 
 }  // namespace shgemm
 }  // namespace mtk

@@ -141,7 +141,8 @@ int shgemm_amd_check_plan_file(const char* path, int* rows);
  * No counterpart in the reference (one problem per call there,
  * include/shgemm/shgemm.hpp:105-119).  Each problem follows shgemm's
  * semantics; all share op_a, op_b, op_c and compute_type.  create plans the
- * problems (tile class 0 small / 1 128x128 / 2 large, and k-slices: from
+ * problems (tile class 0 small / 1 128x128 / 2 large / 3 128x512 for the
+ * prescaled fp16 mode, and k-slices: from
  * tile_class/slices when not NULL, else the plan table), allocates the
  * group's device scratch and captures one HIP graph; run replays it (or, with
  * use_graph = 0, enqueues the same launches one by one) on `stream`.  Runs of
@@ -165,6 +166,23 @@ int shgemm_amd_group_run(shgemm_amd_group* g, void* stream, int use_graph);
  * work), and the total number of work units. Arrays may be NULL. */
 int shgemm_amd_group_plan(shgemm_amd_group* g, int* tile_class, int* slices, size_t* units);
 int shgemm_amd_group_destroy(shgemm_amd_group* g);
+
+/* ---- Batches -------------------------------------------------------------------
+ * No counterpart in the reference: grouped launches of problems that change
+ * from call to call (a serving loop's admitted requests), with no allocation
+ * per call.  create allocates `slots` device regions of `bytes` each and
+ * pinned staging for max_problems descriptors; run plans the problems as
+ * group_create would (plan table's grouped rows), copies the descriptors
+ * into the next slot asynchronously and enqueues the launches on `stream`
+ * (waiting on the host only when that slot's previous batch is still in
+ * flight); SHGEMM_AMD_EINVAL when a batch needs more than `bytes`.  plan
+ * reports the last run's per-problem tile class and slices. */
+typedef struct shgemm_amd_batch shgemm_amd_batch;
+int shgemm_amd_batch_create(shgemm_amd_handle* h, int op_a, int op_b, int op_c, int compute_type,
+                            size_t max_problems, size_t bytes, int slots, shgemm_amd_batch** out);
+int shgemm_amd_batch_run(shgemm_amd_batch* b, const shgemm_amd_problem* problems, size_t count, void* stream);
+int shgemm_amd_batch_plan(shgemm_amd_batch* b, int* tile_class, int* slices, size_t* units);
+int shgemm_amd_batch_destroy(shgemm_amd_batch* b);
 
 /* ---- Sketch matrices -----------------------------------------------------
  * No counterpart in the reference (its tests fill inputs with cuRAND Philox,

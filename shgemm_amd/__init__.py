@@ -92,6 +92,10 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
     L.shgemm_amd_group_run.argtypes = [vp, vp, i32]
     L.shgemm_amd_group_plan.argtypes = [vp, ip, ip, ctypes.POINTER(sz)]
     L.shgemm_amd_group_destroy.argtypes = [vp]
+    L.shgemm_amd_batch_create.argtypes = [vp, i32, i32, i32, i32, sz, sz, i32, ctypes.POINTER(vp)]
+    L.shgemm_amd_batch_run.argtypes = [vp, ctypes.POINTER(Problem), sz, vp]
+    L.shgemm_amd_batch_plan.argtypes = [vp, ip, ip, ctypes.POINTER(sz)]
+    L.shgemm_amd_batch_destroy.argtypes = [vp]
     L.shgemm_amd_gaussian_f16.argtypes = [vp, sz, ctypes.c_uint64, ctypes.c_float, vp]
     L.shgemm_amd_philox_words.argtypes = [vp, sz, ctypes.c_uint64, vp]
     L.shgemm_amd_last_error.restype = ctypes.c_char_p
@@ -324,6 +328,54 @@ class Group:
         if self._g:
             _check(load_library().shgemm_amd_group_destroy(self._g))
             self._g = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class Batch:
+    """Grouped launches of problems that change from call to call, with no
+    allocation per call (shgemm_amd_batch_*): `slots` device regions of
+    `bytes` each hold one batch in flight; run() plans and enqueues one batch
+    of problems (same tuples as Group) on `stream`."""
+
+    def __init__(self, handle, op_a, op_b, compute_type=fp16, op_c=op_n, max_problems=64, bytes=256 << 20,
+                 slots=2):
+        self._b = ctypes.c_void_p()
+        self.max_problems = int(max_problems)
+        self._arr = (Problem * self.max_problems)()
+        self._last = 0
+        _check(load_library().shgemm_amd_batch_create(handle.ptr, int(op_a), int(op_b), int(op_c), int(compute_type),
+                                                      self.max_problems, int(bytes), int(slots),
+                                                      ctypes.byref(self._b)))
+
+    def run(self, problems, stream=None) -> None:
+        n = len(problems)
+        if n > self.max_problems:
+            raise ShgemmError(STATUS_EINVAL, f"{n} problems exceed max_problems {self.max_problems}")
+        keys = ("m", "n", "k", "a", "lda", "b", "ldb", "c", "ldc", "alpha", "beta")
+        for i, q in enumerate(problems):
+            d = dict(zip(keys, q)) if not isinstance(q, dict) else q
+            self._arr[i] = Problem(d["m"], d["n"], d["k"], _dptr(d["a"]), d["lda"], _dptr(d["b"]), d["ldb"],
+                                   _dptr(d["c"]), d["ldc"], float(d.get("alpha", 1.0)), float(d.get("beta", 0.0)))
+        self._last = n
+        _check(load_library().shgemm_amd_batch_run(self._b, self._arr, n, _raw_stream(stream)))
+
+    def plan(self) -> tuple[list[int], list[int], int]:
+        """Per-problem tile class and slice count of the last run, and its work units."""
+        tc = (ctypes.c_int * max(1, self._last))()
+        sl = (ctypes.c_int * max(1, self._last))()
+        units = ctypes.c_size_t()
+        _check(load_library().shgemm_amd_batch_plan(self._b, tc, sl, ctypes.byref(units)))
+        return list(tc)[:self._last], list(sl)[:self._last], units.value
+
+    def destroy(self) -> None:
+        if self._b:
+            _check(load_library().shgemm_amd_batch_destroy(self._b))
+            self._b = ctypes.c_void_p()
 
     def __enter__(self):
         return self

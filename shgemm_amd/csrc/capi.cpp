@@ -20,6 +20,11 @@ struct shgemm_amd_group {
   mtk::shgemm::group_t* g = nullptr;
 };
 
+struct shgemm_amd_batch {
+  mtk::shgemm::batch_t* b = nullptr;
+  std::vector<mtk::shgemm::group_problem> probs;  // reused conversion buffer
+};
+
 namespace {
 thread_local std::string g_last_error;
 
@@ -269,6 +274,67 @@ int shgemm_amd_group_create(shgemm_amd_handle* h, int op_a, int op_b, int op_c, 
       throw;
     }
     *out = g;
+  });
+}
+
+namespace {
+void to_group_problems(const shgemm_amd_problem* problems, size_t count, std::vector<mtk::shgemm::group_problem>& v) {
+  v.resize(count);
+  for (size_t i = 0; i < count; ++i) {
+    const shgemm_amd_problem& q = problems[i];
+    v[i].m = q.m;
+    v[i].n = q.n;
+    v[i].k = q.k;
+    v[i].A = q.a;
+    v[i].lda = q.lda;
+    v[i].B = reinterpret_cast<const half*>(q.b);
+    v[i].ldb = q.ldb;
+    v[i].C = q.c;
+    v[i].ldc = q.ldc;
+    v[i].alpha = q.alpha;
+    v[i].beta = q.beta;
+  }
+}
+}  // namespace
+
+int shgemm_amd_batch_create(shgemm_amd_handle* h, int op_a, int op_b, int op_c, int compute_type,
+                            size_t max_problems, size_t bytes, int slots, shgemm_amd_batch** out) {
+  return guarded([&] {
+    need(h);
+    if (out == nullptr) throw mtk::shgemm::error("out is null");
+    auto* b = new shgemm_amd_batch();
+    try {
+      b->b = mtk::shgemm::create_batch(h->h, to_op(op_a), to_op(op_b), to_tc(compute_type), to_op(op_c),
+                                       max_problems, bytes, slots);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int shgemm_amd_batch_run(shgemm_amd_batch* b, const shgemm_amd_problem* problems, size_t count, void* stream) {
+  return guarded([&] {
+    if (b == nullptr) throw mtk::shgemm::error("batch is null");
+    if (problems == nullptr && count > 0) throw mtk::shgemm::error("problems is null");
+    to_group_problems(problems, count, b->probs);
+    mtk::shgemm::run_batch(b->b, b->probs.data(), count, static_cast<hipStream_t>(stream));
+  });
+}
+
+int shgemm_amd_batch_plan(shgemm_amd_batch* b, int* tile_class, int* slices, size_t* units) {
+  return guarded([&] {
+    if (b == nullptr) throw mtk::shgemm::error("batch is null");
+    mtk::shgemm::batch_plan(b->b, tile_class, slices, units);
+  });
+}
+
+int shgemm_amd_batch_destroy(shgemm_amd_batch* b) {
+  return guarded([&] {
+    if (b == nullptr) return;
+    mtk::shgemm::destroy_batch(b->b);
+    delete b;
   });
 }
 

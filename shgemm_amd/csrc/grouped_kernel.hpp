@@ -69,11 +69,19 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_grouped_kernel(const
 // problem run in a group gives the same bits as the same plan run alone; the
 // k-sliced small-tile plan differs from P0 only in its prefetch depth, which
 // does not change the arithmetic).
+// Class 3 (prescaled fp16 only): 128 x 512 with eight 64 x 128 wavefronts, so
+// a problem with n <= 512 reads each row panel of A once (the 128 x 256 tile
+// reads it twice at n = 512); one workgroup per CU (96 KiB of LDS), two
+// wavefronts per SIMD.  Same k-tile depth and running row exponents as P1, so
+// its problems give P1's bits.  The other modes have no room for it (the BF16
+// stage is 90 KiB, the two-accumulator mode needs 256 accumulator registers).
 template <int MODE>
 struct GroupCfg {
   using C0 = typename ModeCfg<MODE>::P0;
   using C1 = typename ModeCfg<MODE>::SK;
   using C2 = typename ModeCfg<MODE>::P1;
+  using C3 = TileCfg<128, 512, 32, 64, 128, 1, 2>;
+  static constexpr bool kHasC3 = (MODE == MODE_FP16S);
 };
 
 }  // namespace shgemm_amd

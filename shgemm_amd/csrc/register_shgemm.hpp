@@ -61,6 +61,7 @@ inline void put_shgemm_opc(KernelTable& t) {
   put_grouped<OPA, OPB, OPC, MODE, typename G::C0>(t, 0);
   put_grouped<OPA, OPB, OPC, MODE, typename G::C1>(t, 1);
   put_grouped<OPA, OPB, OPC, MODE, typename G::C2>(t, 2);
+  if constexpr (G::kHasC3) put_grouped<OPA, OPB, OPC, MODE, typename G::C3>(t, 3);
   using C = ModeCfg<MODE>;
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, false>(t, V_P1);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0, false>(t, V_P0);

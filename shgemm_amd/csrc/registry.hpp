@@ -36,8 +36,9 @@ enum : int {
 inline bool variant_is_split(int v) { return v == V_SK || v == V_P1SK || v == V_P0SK; }
 inline int variant_level(int v) { return (v == V_P0 || v == V_P0SK) ? LEVEL_P0 : LEVEL_P1; }
 
-// Tile classes of the grouped kernel (small, 128 x 128, the mode's large tile).
-enum : int { kNumGroupClasses = 3 };
+// Tile classes of the grouped kernel (small, 128 x 128, the mode's large tile,
+// and for the prescaled fp16 mode a 128 x 512 tile that covers n up to 512).
+enum : int { kNumGroupClasses = 4 };
 
 struct KernelTable {
   // [op_a][op_b][op_c][mode][variant]

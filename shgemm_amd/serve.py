@@ -8,18 +8,21 @@ shgemm_amd.gaussian_f16), keeps it in HBM for the life of the request, and
 computes Y = A * Omega with shgemm (fp16 compute type, error-corrected).
 
 Continuous batching: the server owns `slots` independent lanes, each a
-library handle bound to its own HIP stream (default priority).  A request
-holds one lane from admission to completion; requests admitted while others
-run are enqueued on other lanes at once, so their kernels overlap on the
-device.  Admission is first come, first served: the oldest waiting request is
-admitted as soon as a lane is free and the in-flight footprint plus its own
-(4mk + 2kl + 4ml bytes) fits the HBM budget.  A request larger than the whole
-budget is rejected when it is submitted.
+library handle bound to its own HIP stream (default priority).  At every
+scheduling step the waiting requests are admitted first come, first served
+while a lane is free and the in-flight footprint plus theirs (4mk + 2kl + 4ml
+bytes each) fits the HBM budget: with max_batch > 1 up to max_batch of them
+go to one free lane together and run as ONE grouped launch of problems of
+different sizes (shgemm_amd.Batch: each request one problem, planned by its
+own shape), otherwise each takes a lane and runs as one shgemm call.  Work
+admitted while other lanes run overlaps with theirs on the device; a lane and
+its requests' budget are freed when its launch completes.  A request larger
+than the whole budget is rejected when it is submitted.
 
 Results do not depend on the mix: Omega depends only on (seed, index), and
 every kernel of the product is deterministic for a given shape (fixed plans,
-fixed-order split-K reduction), so a request returns the same bits alone and
-among others.
+fixed-order split-K reduction; in a batch each problem's plan depends on its
+own shape only), so a request returns the same bits alone and among others.
 
 There is no serving component in the reference; the sketch use case is the
 one its README names (README.md:64-72) and its tests seed inputs with a
@@ -69,7 +72,10 @@ class SketchRequest:
 class GpuLanes:
     """The device side of the server: per lane a handle and a HIP stream."""
 
-    def __init__(self, slots: int, compute_type=sg.fp16, device: str = "cuda:0"):
+    def __init__(self, slots: int, compute_type=sg.fp16, device: str = "cuda:0", max_batch: int = 0,
+                 batch_bytes: int = 1 << 30):
+        """max_batch > 0: every lane also owns a shgemm_amd.Batch for up to
+        max_batch requests with batch_bytes of device scratch per slot."""
         import torch
 
         self.torch = torch
@@ -77,6 +83,7 @@ class GpuLanes:
         self.compute_type = compute_type
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
         self.handles = []
+        self.batches = []
         for s in self.streams:
             h = sg.create()
             sg.set_cuda_stream(h, s)
@@ -84,6 +91,9 @@ class GpuLanes:
             # vs 10.8 ms at 2000 req/s with the whole-CU 256x256 plans)
             sg.set_shared_device(h, slots > 1)
             self.handles.append(h)
+            if max_batch > 0:
+                self.batches.append(sg.Batch(h, sg.op_n, sg.op_n, compute_type, sg.op_n, max_problems=max_batch,
+                                             bytes=batch_bytes))
 
     def reserve(self, max_m: int, max_k: int = 0, max_l: int = 0) -> None:
         """Prepare every lane for requests up to (max_m, max_k, max_l) before
@@ -116,31 +126,68 @@ class GpuLanes:
         ev.record(s)
         return ev
 
+    def launch_batch(self, reqs: list, slot: int):
+        """Omega of every request, then one grouped launch of all of them on
+        the lane's stream (split in halves if the batch's scratch would not
+        fit the lane's batch slots; each problem's plan depends on its shape
+        only, so the bits are the same)."""
+        torch = self.torch
+        s = self.streams[slot]
+        for req in reqs:
+            with torch.cuda.stream(s):
+                req.omega = torch.empty(req.k * req.l, dtype=torch.float16, device=self.device)
+                req.y = torch.empty(req.m * req.l, dtype=torch.float32, device=self.device)
+            sg.gaussian_f16(req.omega, req.k * req.l, req.seed, 1.0 / math.sqrt(req.l), stream=s)
+
+        def run(part):
+            probs = [(r.m, r.l, r.k, r.a, r.lda, r.omega, r.k, r.y, r.m, 1.0, 0.0) for r in part]
+            try:
+                self.batches[slot].run(probs, stream=s)
+            except sg.ShgemmError as e:
+                if e.status != sg.STATUS_EINVAL or len(part) == 1:
+                    raise
+                run(part[:len(part) // 2])
+                run(part[len(part) // 2:])
+
+        run(reqs)
+        ev = torch.cuda.Event()
+        ev.record(s)
+        return ev
+
     @staticmethod
     def finished(token) -> bool:
         return token.query()
 
     def close(self) -> None:
         self.torch.cuda.synchronize(self.device)
+        for b in self.batches:
+            b.destroy()
+        self.batches = []
         for h in self.handles:
             sg.destroy(h)
         self.handles = []
 
 
 class SketchServer:
-    """FIFO admission under an HBM budget onto `slots` concurrent lanes."""
+    """FIFO admission under an HBM budget onto `slots` concurrent lanes; with
+    max_batch > 1 the requests admitted to one lane in a step run as one
+    grouped launch."""
 
     def __init__(self, hbm_budget: float = 200e9, slots: int = 4, lanes=None,
-                 clock: Callable[[], float] = time.perf_counter, keep_omega: bool = False):
+                 clock: Callable[[], float] = time.perf_counter, keep_omega: bool = False, max_batch: int = 1):
         if slots < 1:
             raise ValueError("slots must be >= 1")
+        if max_batch < 1:
+            raise ValueError("max_batch must be >= 1")
         self.budget = int(hbm_budget)
-        self.lanes = lanes if lanes is not None else GpuLanes(slots)
+        self.max_batch = max_batch
+        self.lanes = lanes if lanes is not None else GpuLanes(slots, max_batch=max_batch if max_batch > 1 else 0)
         self.clock = clock
         self.keep_omega = keep_omega
         self.free = collections.deque(range(slots))
         self.queue: collections.deque[SketchRequest] = collections.deque()
-        self.inflight: dict[int, tuple[SketchRequest, object]] = {}
+        self.inflight: dict[int, tuple[list, object]] = {}  # lane -> (requests, completion token)
+        self.batch_sizes: list[int] = []
         self.in_use = 0
         self.peak_in_use = 0
         self.peak_inflight = 0
@@ -156,31 +203,41 @@ class SketchServer:
     def poll(self) -> list[SketchRequest]:
         """Completes every request whose work has finished; frees its lane and budget."""
         done = []
-        for slot, (req, token) in list(self.inflight.items()):
+        for slot, (reqs, token) in list(self.inflight.items()):
             if self.lanes.finished(token):
-                req.done = self.clock()
+                t = self.clock()
                 del self.inflight[slot]
                 self.free.append(slot)
-                self.in_use -= req.footprint
-                if not self.keep_omega:
-                    req.omega = None
-                done.append(req)
+                for req in reqs:
+                    req.done = t
+                    self.in_use -= req.footprint
+                    if not self.keep_omega:
+                        req.omega = None
+                    done.append(req)
         return done
 
     def admit(self) -> int:
-        """Admits waiting requests in arrival order while a lane and budget allow."""
+        """Admits waiting requests in arrival order while a lane and budget
+        allow: up to max_batch of them per free lane, as one launch."""
         n = 0
         while self.queue and self.free and self.in_use + self.queue[0].footprint <= self.budget:
-            req = self.queue.popleft()
             slot = self.free.popleft()
-            req.slot = slot
-            req.admitted = self.clock()
-            self.in_use += req.footprint
+            batch = []
+            while (self.queue and len(batch) < self.max_batch
+                   and self.in_use + self.queue[0].footprint <= self.budget):
+                req = self.queue.popleft()
+                req.slot = slot
+                req.admitted = self.clock()
+                self.in_use += req.footprint
+                self.admit_order.append(req.rid)
+                batch.append(req)
             self.peak_in_use = max(self.peak_in_use, self.in_use)
-            self.inflight[slot] = (req, self.lanes.launch(req, slot))
-            self.peak_inflight = max(self.peak_inflight, len(self.inflight))
-            self.admit_order.append(req.rid)
-            n += 1
+            token = (self.lanes.launch_batch(batch, slot) if self.max_batch > 1
+                     else self.lanes.launch(batch[0], slot))
+            self.inflight[slot] = (batch, token)
+            self.batch_sizes.append(len(batch))
+            self.peak_inflight = max(self.peak_inflight, sum(len(b) for b, _ in self.inflight.values()))
+            n += len(batch)
         return n
 
     def step(self) -> list[SketchRequest]:
@@ -240,6 +297,8 @@ def run_arrivals(server: SketchServer, requests: list[SketchRequest], offsets: l
                           if len(offsets) > 1 and max(offsets) > min(offsets) else float("inf")),
         "flops_per_s": sum(r.flops for r in done) / (last - first),
         "peak_inflight": server.peak_inflight,
+        "mean_batch": sum(server.batch_sizes) / max(1, len(server.batch_sizes)),
+        "max_batch_seen": max(server.batch_sizes, default=0),
         "peak_budget_bytes": server.peak_in_use,
     }
 

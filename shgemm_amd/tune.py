@@ -10,13 +10,16 @@ Two kinds of rows (format in csrc/plans.hpp):
            variant (P0, P1, P2, P2 ring, P1 three-stage, and the split-K
            variants at 2..256 slices) is timed alone; the fastest is kept.
   grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
-           2 large) x k-slices {1, 2, 4} are timed on a group of identical
-           copies large enough to fill the GPU several times over, so the
-           choice is the best throughput inside a busy group.
+           2 large, 3 128x512 in the prescaled fp16 mode) x k-slices {1, 2, 4}
+           are timed on a group of identical copies large enough to fill the
+           GPU several times over, so the choice is the best throughput inside
+           a busy group.
 
 Usage (on a GPU):
   python -m shgemm_amd.tune --out shgemm_amd/plans_gfx950.tsv
   python -m shgemm_amd.tune --quick --out /tmp/plans.tsv     # reduced grid
+  python -m shgemm_amd.tune --single '' --keep shgemm_amd/plans_gfx950.tsv \
+      --out shgemm_amd/plans_gfx950.tsv                       # retune the grouped rows only
 """
 from __future__ import annotations
 
@@ -149,7 +152,7 @@ def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
                 As = [torch.rand(m * k, device="cuda") * 2 - 1 for _ in range(n_a)]
                 Bs = [torch.randn(k * n, device="cuda").half() for _ in range(n_a)]
                 best = None
-                for cls in (0, 1, 2):
+                for cls in GROUP_CLASSES[mode]:
                     for s in GROUP_SLICES:
                         # copies so the group has about GROUP_UNITS work units
                         tiles = _tiles(cls, mode, m, n)
@@ -177,9 +180,13 @@ def tune_grouped(h, mode: str, quick: bool, log) -> list[str]:
 
 
 # Tile shapes of the grouped classes (csrc/grouped_kernel.hpp GroupCfg; the
-# prescaled mode's large tile is 128 x 256, the others' 128 x 128).
+# prescaled mode's large tile is 128 x 256, the others' 128 x 128; class 3,
+# 128 x 512, exists in the prescaled mode only).
+GROUP_CLASSES = {"fp16s": (0, 1, 2, 3), "fp16": (0, 1, 2), "bf16": (0, 1, 2)}
+
+
 def _tiles(cls: int, mode: str, m: int, n: int) -> int:
-    bm, bn = {0: (64, 64), 1: (128, 128), 2: (128, 256 if mode == "fp16s" else 128)}[cls]
+    bm, bn = {0: (64, 64), 1: (128, 128), 2: (128, 256 if mode == "fp16s" else 128), 3: (128, 512)}[cls]
     return -(-m // bm) * -(-n // bn)
 
 
@@ -240,6 +247,8 @@ def main(argv=None) -> int:
     ap.add_argument("--single", default="fp16s:NNN,fp16s:TNN,fp16s:TTT,fp16:NNN,bf16:NNN",
                     help="mode:ops list for single-call rows ('' for none)")
     ap.add_argument("--grouped", default="fp16s,bf16", help="modes for grouped rows ('' for none)")
+    ap.add_argument("--keep", default="", help="existing table whose rows of the kinds and modes not tuned "
+                                               "now are kept")
     args = ap.parse_args(argv)
     torch = _torch()
     t0 = time.time()
@@ -260,6 +269,19 @@ def main(argv=None) -> int:
             rows += tune_grouped(h, mode, args.quick, log)
     finally:
         sg.destroy(h)
+    tuned = {("single", item.split(":")[0], item.split(":")[1]) for item in filter(None, args.single.split(","))}
+    tuned |= {("grouped", mode, None) for mode in filter(None, args.grouped.split(","))}
+    kept = []
+    if args.keep:
+        for line in Path(args.keep).read_text().splitlines():
+            tok = line.split()
+            if not tok or tok[0].startswith("#"):
+                continue
+            if (tok[0], tok[1], "".join(tok[2:5])) in tuned or (tok[0], tok[1], None) in tuned:
+                continue
+            kept.append(line)
+        log(f"kept {len(kept)} rows of {args.keep}")
+    rows = kept + rows
     props = torch.cuda.get_device_properties(0)
     head = [
         "# shgemm_amd plan table (kind mode op_a op_b op_c log2m log2n log2k choice slices time_us)",
@@ -269,7 +291,7 @@ def main(argv=None) -> int:
         "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage,"
         " 8 P2 on pre-split A);"
         " time_us per call",
-        "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large); time_us per problem in a full group",
+        "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large, 3 128x512); time_us per problem in a full group",
     ]
     out = Path(args.out)
     out.write_text("\n".join(head + rows) + "\n")

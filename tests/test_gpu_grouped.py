@@ -211,7 +211,7 @@ def test_group_rejects_invalid(h):
         sg.Group(h, 0, 0, [good.tup(), tuple(bad)], 0)
     assert ei.value.status == sg.STATUS_EINVAL
     with pytest.raises(sg.ShgemmError) as ei:
-        sg.Group(h, 0, 0, [good.tup()], 0, 0, [3], [1])
+        sg.Group(h, 0, 0, [good.tup()], 0, 0, [4], [1])
     assert ei.value.status == sg.STATUS_EINVAL
     with pytest.raises(sg.ShgemmError) as ei:
         sg.Group(h, 0, 0, [], 0)

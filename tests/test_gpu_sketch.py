@@ -262,3 +262,61 @@ def test_server_stated_distribution_alone_mix_and_budget(torch_mod):
     lanes.close()
     del pools
     torch.cuda.empty_cache()
+
+
+def test_batched_server_alone_and_in_mix(torch_mod):
+    """max_batch = 4: the requests admitted together run as one grouped
+    launch.  Requests from the stated distribution (A blocks of four
+    32768 x 16384 pools) give the same bits alone (batches of one) and in a
+    burst where batches of several form; sampled rows stay within 2x of
+    SGEMM's FP64 error."""
+    import shgemm_amd as sg
+    from shgemm_amd.serve import GpuLanes, SketchServer, poisson_offsets, run_arrivals
+
+    torch = torch_mod
+    ROWS, COLS, NPOOL, NREQ = 32768, 16384, 4, 20
+    g = torch.Generator(device="cuda")
+    g.manual_seed(23)
+    pools = [torch.rand(ROWS * COLS, device="cuda", generator=g) * 2 - 1 for _ in range(NPOOL)]
+
+    def draw():
+        return _stated_requests(torch, pools, ROWS, COLS, NREQ, np.random.default_rng(44))
+
+    lanes = GpuLanes(2, max_batch=4)
+    alone_reqs = draw()
+    srv = SketchServer(hbm_budget=200e9, slots=2, lanes=lanes, keep_omega=True, max_batch=4)
+    alone = []
+    for r in alone_reqs:
+        srv.submit(r)
+        srv.drain()
+        torch.cuda.synchronize()
+        alone.append(r.y.cpu().numpy().copy())
+    assert srv.batch_sizes == [1] * NREQ
+    h = sg.create()
+    rng = np.random.default_rng(9)
+    for r, y in list(zip(alone_reqs, alone))[:6]:
+        pool = next(p for p in pools if 0 <= (r.a - p.data_ptr()) // 4 < ROWS * COLS)
+        off = (r.a - pool.data_ptr()) // 4
+        r0, c0 = off % ROWS, off // ROWS
+        rows = np.sort(rng.choice(r.m, min(r.m, 16), replace=False))
+        A_rows = pool.view(COLS, ROWS)[c0:c0 + r.k][:, torch.from_numpy(rows + r0).cuda()].t().double().cpu().numpy()
+        ref = A_rows @ r.omega.view(r.l, r.k).t().double().cpu().numpy()
+        Ys = torch.empty(r.m * r.l, device="cuda")
+        sg.sgemm_fp32(h, 0, 0, r.m, r.l, r.k, 1.0, r.a, r.lda, r.omega.float(), r.k, 0.0, Ys, r.m)
+        Ys_rows = Ys.view(r.l, r.m).t()[torch.from_numpy(rows).cuda()].double().cpu().numpy()
+        Y = y.reshape(r.l, r.m).T[rows].astype(np.float64)
+        e = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
+        es = np.linalg.norm(Ys_rows - ref) / np.linalg.norm(ref)
+        assert e <= 2.0 * es, (r.rid, e, es)
+    sg.destroy(h)
+
+    mix = draw()
+    srv = SketchServer(hbm_budget=200e9, slots=2, lanes=lanes, max_batch=4)
+    st = run_arrivals(srv, mix, poisson_offsets(NREQ, 50000.0, np.random.default_rng(5)))
+    torch.cuda.synchronize()
+    assert st["requests"] == NREQ and st["max_batch_seen"] >= 2   # batches did form
+    for r, y in zip(mix, alone):
+        assert np.array_equal(r.y.cpu().numpy().view(np.uint32), y.view(np.uint32)), r.rid
+    lanes.close()
+    del pools
+    torch.cuda.empty_cache()

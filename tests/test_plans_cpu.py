@@ -25,7 +25,7 @@ def test_parses_good_table(tmp_path):
 
 @pytest.mark.parametrize("line,why", [
     ("single fp16s N N N 10 10 10 10 1", "variant 10"),
-    ("grouped fp16s N N N 10 10 10 3 1", "class 3"),
+    ("grouped fp16s N N N 10 10 10 4 1", "class 4"),
     ("single fp16x N N N 10 10 10 0 1", "mode"),
     ("single fp16s N X N 10 10 10 0 1", "op"),
     ("single fp16s N N N 10 10 10 0 0", "slices 0"),

@@ -21,6 +21,10 @@ class FakeLanes:
         self.launched.append((req.rid, slot))
         return {"left": self.polls(req)}
 
+    def launch_batch(self, reqs, slot):
+        self.launched.append((tuple(r.rid for r in reqs), slot))
+        return {"left": max(self.polls(r) for r in reqs)}
+
     @staticmethod
     def finished(token):
         token["left"] -= 1
@@ -117,3 +121,41 @@ def test_poisson_and_shapes():
     assert set(ls) <= {64, 128, 256, 512, 1024}
     # log-uniform: the median is near the geometric mean of the range
     assert abs(math.log(np.median(ms)) - 0.5 * (math.log(1024) + math.log(32768))) < 0.15
+
+
+def test_batched_admission_fifo_budget_and_completion():
+    """max_batch > 1: the requests waiting at a step go to one free lane as
+    one launch (up to max_batch, within the budget, in arrival order); a
+    batch's requests complete together and free their budget then."""
+    reqs = [_req(i, m=100 * (1 + i % 3), k=200, l=64) for i in range(30)]
+    budget = 5 * max(r.footprint for r in reqs)
+    lanes = FakeLanes(lambda r: 2 + r.rid % 4)
+    srv = SketchServer(hbm_budget=budget, slots=2, lanes=lanes, clock=FakeClock(), max_batch=4)
+    for r in reqs:
+        srv.submit(r)
+    done = []
+    while not srv.idle():
+        done += srv.poll()
+        srv.admit()
+        assert srv.in_use <= budget
+        assert len(srv.inflight) <= 2
+        assert all(len(b) <= 4 for b, _ in srv.inflight.values())
+    assert sorted(r.rid for r in done) == list(range(30))
+    order = [rid for batch, _ in lanes.launched for rid in batch]
+    assert order == list(range(30))                         # first come, first served
+    assert max(len(b) for b, _ in lanes.launched) == 4      # a backlog is batched
+    # every request of a launch completes at the same time
+    for batch, _ in lanes.launched:
+        t = {next(r for r in done if r.rid == rid).done for rid in batch}
+        assert len(t) == 1
+    st_sizes = srv.batch_sizes
+    assert sum(st_sizes) == 30 and len(st_sizes) == len(lanes.launched)
+
+
+def test_batching_off_launches_one_request_per_lane():
+    lanes = FakeLanes(lambda r: 1)
+    srv = SketchServer(hbm_budget=1e12, slots=3, lanes=lanes, clock=FakeClock())
+    for i in range(7):
+        srv.submit(_req(i))
+    srv.drain()
+    assert all(isinstance(rid, int) for rid, _ in lanes.launched)

@@ -140,14 +140,22 @@ def test_f16_mfma_accumulation_bound_and_bias(tk):
     D, exact, _ = _accumulate_cases(tk, A2, B, zero)
     assert np.all(np.abs(D - exact) <= _ulp32(exact))
 
-    for a, b, c, same_sign in ((A, B, C, False), (np.abs(A), np.abs(B), np.abs(C), True),
-                               (np.abs(A), np.abs(B), zero, True)):
+    for name, a, b, c, same_sign in (("16 products + c, mixed signs", A, B, C, False),
+                                      ("16 products + c, all positive", np.abs(A), np.abs(B), np.abs(C), True),
+                                      ("16 positive products, c = 0", np.abs(A), np.abs(B), zero, True)):
         D, exact, big = _accumulate_cases(tk, a, b, c)
         err = D - exact
-        assert np.all(np.abs(err) <= 8.0 * _ulp32(big) + _ulp32(exact)), np.max(np.abs(err) / _ulp32(big))
-        assert np.mean(D != exact.astype(np.float32).astype(np.float64)) > 0.2
+        rel_big, rel_res = err / _ulp32(big), err / _ulp32(exact)
+        off_rne = np.mean(D != exact.astype(np.float32).astype(np.float64))
+        nz = err != 0
+        toward0 = np.mean(np.sign(err[nz]) == -np.sign(exact[nz])) if nz.any() else 0.0
+        print(f"{name}: {D.size} outputs, differ from round-to-nearest-even {off_rne:.3f}, "
+              f"max |err| {np.max(np.abs(rel_big)):.2f} ulp(largest operand), mean err {np.mean(rel_res):+.3f} "
+              f"ulp(result), toward zero {toward0:.3f} of nonzero errors", flush=True)
+        assert np.all(np.abs(err) <= 8.0 * _ulp32(big) + _ulp32(exact)), np.max(np.abs(rel_big))
+        assert off_rne > 0.2
         if same_sign:
-            assert np.mean(err / _ulp32(exact)) < -0.05
+            assert np.mean(rel_res) < -0.05
 
 
 def _split_inputs(rng, npairs):

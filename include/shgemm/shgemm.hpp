@@ -159,7 +159,9 @@ void set_shared_device(shgemmHandle_t& handle, const bool on);
 //   the 256x256 tile runs without a split (prescaled fp16, k a multiple of
 //   64; otherwise it runs as 5), 9 the small tile on whole-row exponents
 //   from a pass over A instead of running ones (prescaled fp16; otherwise
-//   it runs as 0).
+//   it runs as 0), 10 the deep k-sliced 128x128x64 tile (eight wavefronts,
+//   two k-tiles of loads in flight; prescaled fp16, split over k like 2;
+//   otherwise it runs as 2).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.
@@ -169,7 +171,8 @@ enum {
   plan_variant_bd = 7,
   plan_variant_ps = 8,
   plan_variant_p0w = 9,
-  num_plan_variants = 10
+  plan_variant_skd = 10,
+  num_plan_variants = 11
 };
 enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
 struct plan_t {

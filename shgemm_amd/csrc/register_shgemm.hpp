@@ -69,7 +69,10 @@ inline void put_shgemm_opc(KernelTable& t) {
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P1, true>(t, V_P1SK);
   put_shgemm<OPA, OPB, OPC, MODE, typename C::P0SK, true>(t, V_P0SK);
   if constexpr (C::kHasP2) put_shgemm<OPA, OPB, OPC, MODE, typename C::P2, false>(t, V_P2);
-  if constexpr (MODE == MODE_FP16S) fill_shgemm<OPA, OPB, OPC, MODE, typename C::P0W, false>(t.shgemm_p0w[OPA][OPB][OPC]);
+  if constexpr (MODE == MODE_FP16S) {
+    fill_shgemm<OPA, OPB, OPC, MODE, typename C::P0W, false>(t.shgemm_p0w[OPA][OPB][OPC]);
+    fill_shgemm<OPA, OPB, OPC, MODE, typename C::SKD, true>(t.shgemm_skd[OPA][OPB][OPC]);
+  }
   if constexpr (MODE == MODE_FP16S && OPA == OP_N && OPB == OP_N) put_ring<OPC, typename C::P2>(t);
 }
 

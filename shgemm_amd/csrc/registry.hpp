@@ -51,6 +51,8 @@ struct KernelTable {
   KernelEntry shgemm_ps[2];
   // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
   KernelEntry shgemm_p0w[2][2][2];
+  // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]
+  KernelEntry shgemm_skd[2][2][2];
   // persistent grouped kernels: [op_a][op_b][op_c][mode][tile class]
   KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,

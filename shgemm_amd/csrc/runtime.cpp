@@ -178,6 +178,21 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
       pl.entry = &e;
       pl.note = " whole-row exponents";
     }
+  } else if (requested == plan_variant_skd) {
+    const auto& e = st->table.shgemm_skd[oa][ob][oc];
+    if (mode == shgemm_amd::MODE_FP16S && e.fn != nullptr) {
+      const int64_t smax = max_slices(e, m, n, k, st->ws_bytes);
+      pl.slices = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(2, slices), smax));
+      if (pl.slices > 1) {
+        pl.variant = shgemm_amd::V_SK;
+        pl.entry = &e;
+        pl.note = " deep k-tile";
+      } else {
+        pl.variant = shgemm_amd::V_P1;  // k too short to split: the unsplit large tile, as for plan 2
+      }
+    } else {
+      return resolve_plan(st, tab, shgemm_amd::V_SK, slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b, lda, ldb);
+    }
   } else if (requested >= 0 && requested < shgemm_amd::kNumVariants && tab[requested].fn != nullptr) {
     pl.variant = requested;
     if (shgemm_amd::variant_is_split(requested)) {
@@ -493,7 +508,8 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
                               (int64_t)k, true, true, lda, ldb);
   plan_t out;
   out.variant = pl.presplit ? plan_variant_ps
-                : (pl.entry != nullptr && (pl.requested == plan_variant_bd || pl.requested == plan_variant_p0w)
+                : (pl.entry != nullptr && (pl.requested == plan_variant_bd || pl.requested == plan_variant_p0w ||
+                                          pl.requested == plan_variant_skd)
                        ? pl.requested
                        : pl.variant);
   out.slices = pl.slices;

@@ -7,8 +7,10 @@ measurements on the device at hand.
 
 Two kinds of rows (format in csrc/plans.hpp):
   single   one shgemm call at shape point (2^lm, 2^ln, 2^lk): every plan
-           variant (P0, P1, P2, P2 ring, P1 three-stage, and the split-K
-           variants at 2..256 slices) is timed alone; the fastest is kept.
+           variant (P0, P1, P2, P2 ring, P1 three-stage, P2 on the pre-split
+           A, P0 on whole-row exponents, and the split-K variants, among them
+           the deep 128x128x64 tile, at 2..256 slices) is timed alone; the
+           fastest is kept.
   grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
            2 large, 3 128x512 in the prescaled fp16 mode) x k-slices {1, 2, 4}
            are timed on a group of identical copies large enough to fill the
@@ -102,20 +104,50 @@ def single_shapes(quick: bool):
                 yield m, n, k
 
 
-def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
+SPLIT_SLICES = (2, 4, 8, 16, 32, 64, 128, 256)
+
+
+def _kept_single(path) -> dict:
+    """(mode, ops, lm, ln, lk) -> (choice, slices, row text) of the single rows of a table."""
+    out = {}
+    for line in Path(path).read_text().splitlines():
+        tok = line.split()
+        if len(tok) >= 10 and tok[0] == "single":
+            out[(tok[1], "".join(tok[2:5]), int(tok[5]), int(tok[6]), int(tok[7]))] = (int(tok[8]), int(tok[9]), line)
+    return out
+
+
+def tune_single(h, mode: str, ops: str, quick: bool, log, only=None, kept=None) -> list[str]:
+    """Rows for one mode and layout.  With `only` (plan ids), each shape point
+    times just those plans (the deep k-sliced tile only where m*n <= 2^24, as
+    it loses wherever the output tiles alone fill the GPU) against the kept
+    table's choice for the point, and keeps the faster."""
     torch = _torch()
     ct, scaling = MODES[mode]
     oa, ob, oc = (OPS[c] for c in ops)
     sg.set_exponent_scaling(h, scaling)
     rows = []
     for m, n, k in single_shapes(quick):
+        lm, ln, lk = (int(round(math.log2(x))) for x in (m, n, k))
+        if only is None:
+            candidates = [(v, 1) for v in (0, 1, 5, 6, 7, 8, 9)] + [(v, s) for v in (2, 3, 4, 10) for s in SPLIT_SLICES]
+        else:
+            new = []
+            for v in only:
+                if v == 10 and m * n > (1 << 24):
+                    continue
+                new += [(v, s) for s in SPLIT_SLICES] if v in (2, 3, 4, 10) else [(v, 1)]
+            old = kept.get((mode, ops, lm, ln, lk))
+            if not new:
+                if old is not None:
+                    rows.append(old[2])  # nothing new to time at this point
+                continue
+            candidates = ([old[:2]] if old is not None else []) + new
         a_rows, b_rows, c_rows = (m if oa == sg.op_n else k), (k if ob == sg.op_n else n), (m if oc == sg.op_n else n)
         A = torch.rand(m * k, device="cuda") * 2 - 1
         B = torch.randn(k * n, device="cuda").half()
         C = torch.empty(m * n, device="cuda")
         seen, best = set(), None
-        candidates = [(v, 1) for v in (0, 1, 5, 6, 7, 8, 9)] + [(v, s) for v in (2, 3, 4)
-                                                                for s in (2, 4, 8, 16, 32, 64, 128, 256)]
         for v, s in candidates:
             sg.force_plan(h, v, s)
             eff = sg.query_plan(h, oa, ob, m, n, k, ct, oc)[:2]
@@ -128,7 +160,6 @@ def tune_single(h, mode: str, ops: str, quick: bool, log) -> list[str]:
                 best = (t, eff)
         sg.force_plan(h, -1, 1)
         t, (v, s) = best
-        lm, ln, lk = (int(round(math.log2(x))) for x in (m, n, k))
         rows.append(f"single\t{mode}\t{ops[0]}\t{ops[1]}\t{ops[2]}\t{lm}\t{ln}\t{lk}\t{v}\t{s}\t{t * 1e3:.1f}")
         log(f"single {mode} {ops} m={m} n={n} k={k}: {sg.PLAN_VARIANTS[v]} x{s} {t * 1e3:.1f} us "
             f"({2.0 * m * n * k / t / 1e9:.1f} TFLOP/s, {len(seen)} plans)")
@@ -249,6 +280,8 @@ def main(argv=None) -> int:
     ap.add_argument("--grouped", default="fp16s,bf16", help="modes for grouped rows ('' for none)")
     ap.add_argument("--keep", default="", help="existing table whose rows of the kinds and modes not tuned "
                                                "now are kept")
+    ap.add_argument("--only", default="", help="comma list of plan ids: time only these against each point's "
+                                               "choice in the --keep table (single rows)")
     args = ap.parse_args(argv)
     torch = _torch()
     t0 = time.time()
@@ -258,13 +291,17 @@ def main(argv=None) -> int:
         return 0
     if not args.out:
         ap.error("--out is required unless --sweep-tiles is given")
+    only = [int(x) for x in args.only.split(",")] if args.only else None
+    if only is not None and not args.keep:
+        ap.error("--only needs --keep (the table whose choices the plans are timed against)")
+    kept_single = _kept_single(args.keep) if only is not None else None
     h = sg.create()
     sg.load_plans(h, None)  # measure from the cost model's state, not an older table
     rows = []
     try:
         for item in filter(None, args.single.split(",")):
             mode, ops = item.split(":")
-            rows += tune_single(h, mode, ops, args.quick, log)
+            rows += tune_single(h, mode, ops, args.quick, log, only, kept_single)
         for mode in filter(None, args.grouped.split(",")):
             rows += tune_grouped(h, mode, args.quick, log)
     finally:
@@ -289,10 +326,14 @@ def main(argv=None) -> int:
         f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
         f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
         "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage,"
-        " 8 P2 on pre-split A);"
+        " 8 P2 on pre-split A, 9 P0 on whole-row exponents, 10 deep 128x128x64 split-K);"
         " time_us per call",
         "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large, 3 128x512); time_us per problem in a full group",
     ]
+    if args.keep:
+        prev = [l for l in Path(args.keep).read_text().splitlines() if l.startswith(("# made by:", "# date:", "# kept"))]
+        head += ["# kept rows (not re-measured, or re-measured only against --only) come from: " + l[2:]
+                 if not l.startswith("# kept") else l for l in prev]
     out = Path(args.out)
     out.write_text("\n".join(head + rows) + "\n")
     n = sg.check_plan_file(out)

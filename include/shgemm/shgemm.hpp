@@ -161,7 +161,8 @@ void set_shared_device(shgemmHandle_t& handle, const bool on);
 //   from a pass over A instead of running ones (prescaled fp16; otherwise
 //   it runs as 0), 10 the deep k-sliced 128x128x64 tile (eight wavefronts,
 //   two k-tiles of loads in flight; prescaled fp16, split over k like 2;
-//   otherwise it runs as 2).
+//   otherwise it runs as 2), 11 the large tile as four 64x128 wavefronts,
+//   split over k like 3 (prescaled fp16; otherwise it runs as 3).
 // shgemm picks, in this order: a forced plan, a fixed kernel level, the
 // nearest entry of the measured plan table (written by the autotuner,
 // shgemm_amd/tune.py), and otherwise a cost model.
@@ -172,7 +173,8 @@ enum {
   plan_variant_ps = 8,
   plan_variant_p0w = 9,
   plan_variant_skd = 10,
-  num_plan_variants = 11
+  plan_variant_wsk = 11,
+  num_plan_variants = 12
 };
 enum plan_source { plan_cost_model = 0, plan_table = 1, plan_forced = 2, plan_fixed_level = 3 };
 struct plan_t {

@@ -241,7 +241,7 @@ def sgemm_fp32(handle, op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
 
 # ---- kernel plans -------------------------------------------------------------
 PLAN_VARIANTS = {0: "P0", 1: "P1", 2: "SK128", 3: "P1-SK", 4: "P0-SK", 5: "P2", 6: "P2-ring", 7: "P1-3stage",
-                 8: "P2-presplit", 9: "P0-rowexp", 10: "SK-deep"}
+                 8: "P2-presplit", 9: "P0-rowexp", 10: "SK-deep", 11: "P1W-SK"}
 PLAN_SOURCES = {0: "cost model", 1: "table", 2: "forced", 3: "fixed level"}
 
 

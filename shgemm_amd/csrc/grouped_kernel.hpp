@@ -68,7 +68,10 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_grouped_kernel(const
 // the mode's large tile (the single-call P0, SK and P1 configurations, so a
 // problem run in a group gives the same bits as the same plan run alone; the
 // k-sliced small-tile plan differs from P0 only in its prefetch depth, which
-// does not change the arithmetic).
+// does not change the arithmetic).  In the prescaled fp16 mode the large
+// class is the 128 x 256 tile as four 64 x 128 wavefronts (plan 11's tile:
+// P1's k order and exponents, so P1's bits), two workgroups per CU: the
+// 512-problem batch ran 12.8 against 13.5 ms with the eight-wavefront form.
 // Class 3 (prescaled fp16 only): 128 x 512 with eight 64 x 128 wavefronts, so
 // a problem with n <= 512 reads each row panel of A once (the 128 x 256 tile
 // reads it twice at n = 512); one workgroup per CU (96 KiB of LDS), two
@@ -79,7 +82,7 @@ template <int MODE>
 struct GroupCfg {
   using C0 = typename ModeCfg<MODE>::P0;
   using C1 = typename ModeCfg<MODE>::SK;
-  using C2 = typename ModeCfg<MODE>::P1;
+  using C2 = std::conditional_t<MODE == MODE_FP16S, typename ModeCfg<MODE_FP16S>::P1W, typename ModeCfg<MODE>::P1>;
   using C3 = TileCfg<128, 512, 32, 64, 128, 1, 2>;
   static constexpr bool kHasC3 = (MODE == MODE_FP16S);
 };

@@ -72,7 +72,7 @@ bool parse_plan_file(const std::string& path, std::vector<PlanRow>& out, std::st
       err = where + "bad mode or operation";
       return false;
     }
-    const int max_choice = r.kind == PLAN_SINGLE ? 10 : 3;
+    const int max_choice = r.kind == PLAN_SINGLE ? 11 : 3;
     if (!parse_int(tok[5], 0, 31, r.lm) || !parse_int(tok[6], 0, 31, r.ln) || !parse_int(tok[7], 0, 31, r.lk) ||
         !parse_int(tok[8], 0, max_choice, r.choice) || !parse_int(tok[9], 1, 256, r.slices)) {
       err = where + "bad shape, choice or slices";

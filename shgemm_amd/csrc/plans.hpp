@@ -2,7 +2,7 @@
 // mode and operand layout, the kernel choice the autotuner (tune.py) found
 // fastest.  One text row per point:
 //   kind mode op_a op_b op_c log2m log2n log2k choice slices [time_us]
-// kind   single  -- one shgemm call: choice = plan variant 0..10
+// kind   single  -- one shgemm call: choice = plan variant 0..11
 //        grouped -- one problem inside a grouped launch: choice = tile class 0..3
 // mode   fp16 (two accumulators), fp16s (prescaled), bf16 (the tf32 type)
 // Lines starting with '#' are comments.

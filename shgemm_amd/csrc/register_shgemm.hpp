@@ -72,6 +72,7 @@ inline void put_shgemm_opc(KernelTable& t) {
   if constexpr (MODE == MODE_FP16S) {
     fill_shgemm<OPA, OPB, OPC, MODE, typename C::P0W, false>(t.shgemm_p0w[OPA][OPB][OPC]);
     fill_shgemm<OPA, OPB, OPC, MODE, typename C::SKD, true>(t.shgemm_skd[OPA][OPB][OPC]);
+    fill_shgemm<OPA, OPB, OPC, MODE, typename C::P1W, true>(t.shgemm_wsk[OPA][OPB][OPC]);
   }
   if constexpr (MODE == MODE_FP16S && OPA == OP_N && OPB == OP_N) put_ring<OPC, typename C::P2>(t);
 }

@@ -53,6 +53,8 @@ struct KernelTable {
   KernelEntry shgemm_p0w[2][2][2];
   // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]
   KernelEntry shgemm_skd[2][2][2];
+  // prescaled fp16 large tile of four 64 x 128 wavefronts, k-sliced (plan 11): [op_a][op_b][op_c]
+  KernelEntry shgemm_wsk[2][2][2];
   // persistent grouped kernels: [op_a][op_b][op_c][mode][tile class]
   KernelEntry grouped[2][2][2][kNumModes][kNumGroupClasses];
   // Candidate large-tile configurations of the fp16 op_n/op_n/C-op_n kernel,

@@ -178,20 +178,24 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
       pl.entry = &e;
       pl.note = " whole-row exponents";
     }
-  } else if (requested == plan_variant_skd) {
-    const auto& e = st->table.shgemm_skd[oa][ob][oc];
+  } else if (requested == plan_variant_skd || requested == plan_variant_wsk) {
+    // prescaled fp16 k-sliced tiles with their own kernel tables; in the
+    // other modes plan 10 runs as 2 and plan 11 as 3
+    const bool deep = requested == plan_variant_skd;
+    const auto& e = deep ? st->table.shgemm_skd[oa][ob][oc] : st->table.shgemm_wsk[oa][ob][oc];
+    const int base = deep ? shgemm_amd::V_SK : shgemm_amd::V_P1SK;
     if (mode == shgemm_amd::MODE_FP16S && e.fn != nullptr) {
       const int64_t smax = max_slices(e, m, n, k, st->ws_bytes);
       pl.slices = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(2, slices), smax));
       if (pl.slices > 1) {
-        pl.variant = shgemm_amd::V_SK;
+        pl.variant = base;
         pl.entry = &e;
-        pl.note = " deep k-tile";
+        pl.note = deep ? " deep k-tile" : " 64x128 wavefronts";
       } else {
-        pl.variant = shgemm_amd::V_P1;  // k too short to split: the unsplit large tile, as for plan 2
+        pl.variant = shgemm_amd::V_P1;  // k too short to split: the unsplit large tile, as for plans 2 and 3
       }
     } else {
-      return resolve_plan(st, tab, shgemm_amd::V_SK, slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b, lda, ldb);
+      return resolve_plan(st, tab, base, slices, mode, oa, ob, oc, m, n, k, vec_a, vec_b, lda, ldb);
     }
   } else if (requested >= 0 && requested < shgemm_amd::kNumVariants && tab[requested].fn != nullptr) {
     pl.variant = requested;
@@ -509,7 +513,7 @@ plan_t query_plan(const shgemmHandle_t handle, const operation_t op_a, const ope
   plan_t out;
   out.variant = pl.presplit ? plan_variant_ps
                 : (pl.entry != nullptr && (pl.requested == plan_variant_bd || pl.requested == plan_variant_p0w ||
-                                          pl.requested == plan_variant_skd)
+                                          pl.requested == plan_variant_skd || pl.requested == plan_variant_wsk)
                        ? pl.requested
                        : pl.variant);
   out.slices = pl.slices;

@@ -507,6 +507,12 @@ struct ModeCfg<MODE_FP16S> {
   // 128 x 128 x 2^20 at 256 slices: 165 us against 216 us for the small
   // tile's k-sliced plan at 128 slices.
   using SKD = TileCfg<128, 128, 64, 64, 32, 2, 2>;
+  // The large tile with four 64 x 128 wavefronts (plan 11, k-sliced): twice
+  // the MFMA work per fragment read of the eight 64 x 64 wavefronts, two
+  // workgroups per CU.  Same per-element k order and exponents as P1, so
+  // the same bits; the sketch shape 16384 x 256 x 16384 in 4 slices ran
+  // 335 us against 412 us for P1-SK on one box.
+  using P1W = TileCfg<128, 256, 32, 64, 128, 1, 2>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 2, 1, true>;
   static constexpr bool kHasP2 = true;
 };

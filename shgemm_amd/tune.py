@@ -9,8 +9,8 @@ Two kinds of rows (format in csrc/plans.hpp):
   single   one shgemm call at shape point (2^lm, 2^ln, 2^lk): every plan
            variant (P0, P1, P2, P2 ring, P1 three-stage, P2 on the pre-split
            A, P0 on whole-row exponents, and the split-K variants, among them
-           the deep 128x128x64 tile, at 2..256 slices) is timed alone; the
-           fastest is kept.
+           the deep 128x128x64 tile and the large tile of 64x128 wavefronts,
+           at 2..256 slices) is timed alone; the fastest is kept.
   grouped  one problem inside a grouped launch: tile class (0 small, 1 128x128,
            2 large, 3 128x512 in the prescaled fp16 mode) x k-slices {1, 2, 4}
            are timed on a group of identical copies large enough to fill the
@@ -119,8 +119,8 @@ def _kept_single(path) -> dict:
 
 def tune_single(h, mode: str, ops: str, quick: bool, log, only=None, kept=None) -> list[str]:
     """Rows for one mode and layout.  With `only` (plan ids), each shape point
-    times just those plans (the deep k-sliced tile only where m*n <= 2^24, as
-    it loses wherever the output tiles alone fill the GPU) against the kept
+    times just those plans (the k-sliced tiles 10 and 11 only where m*n <=
+    2^24, as they lose wherever the output tiles alone fill the GPU) against the kept
     table's choice for the point, and keeps the faster."""
     torch = _torch()
     ct, scaling = MODES[mode]
@@ -130,13 +130,13 @@ def tune_single(h, mode: str, ops: str, quick: bool, log, only=None, kept=None) 
     for m, n, k in single_shapes(quick):
         lm, ln, lk = (int(round(math.log2(x))) for x in (m, n, k))
         if only is None:
-            candidates = [(v, 1) for v in (0, 1, 5, 6, 7, 8, 9)] + [(v, s) for v in (2, 3, 4, 10) for s in SPLIT_SLICES]
+            candidates = [(v, 1) for v in (0, 1, 5, 6, 7, 8, 9)] + [(v, s) for v in (2, 3, 4, 10, 11) for s in SPLIT_SLICES]
         else:
             new = []
             for v in only:
-                if v == 10 and m * n > (1 << 24):
+                if v in (10, 11) and m * n > (1 << 24):
                     continue
-                new += [(v, s) for s in SPLIT_SLICES] if v in (2, 3, 4, 10) else [(v, 1)]
+                new += [(v, s) for s in SPLIT_SLICES] if v in (2, 3, 4, 10, 11) else [(v, 1)]
             old = kept.get((mode, ops, lm, ln, lk))
             if not new:
                 if old is not None:
@@ -326,7 +326,7 @@ def main(argv=None) -> int:
         f"# made by: python -m shgemm_amd.tune {' '.join(argv if argv is not None else sys.argv[1:])}",
         f"# date: {datetime.datetime.now(datetime.timezone.utc).strftime('%Y-%m-%d %H:%M UTC')}",
         "# single: choice = plan variant (0 P0, 1 P1, 2 SK128, 3 P1-SK, 4 P0-SK, 5 P2 256x256, 6 P2 ring, 7 P1 three-stage,"
-        " 8 P2 on pre-split A, 9 P0 on whole-row exponents, 10 deep 128x128x64 split-K);"
+        " 8 P2 on pre-split A, 9 P0 on whole-row exponents, 10 deep 128x128x64 split-K, 11 large tile of 64x128 wavefronts split-K);"
         " time_us per call",
         "# grouped: choice = tile class (0 64x64, 1 128x128, 2 large, 3 128x512); time_us per problem in a full group",
     ]

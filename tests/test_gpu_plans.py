@@ -63,10 +63,11 @@ def test_forced_split_is_clamped(h):
     sg.force_plan(h, -1, 1)
 
 
-def test_deep_split_plan_resolves(h):
+def test_prescaled_split_plans_resolve(h):
     """Plan 10 (the deep 128x128x64 k-sliced tile) runs in the prescaled fp16
     mode, as plan 2 in the others, and unsplit (plan 1) where k is too short;
-    its slices keep at least 4 k-tiles of 64."""
+    its slices keep at least 4 k-tiles of 64.  Plan 11 (the large tile of
+    64x128 wavefronts, k-sliced) likewise, as plan 3 in the other modes."""
     import shgemm_amd as sg
 
     sg.force_plan(h, 10, 256)
@@ -78,6 +79,13 @@ def test_deep_split_plan_resolves(h):
         assert sg.query_plan(h, 0, 0, 128, 128, 1 << 20, sg.fp16)[0] == 2
         sg.set_exponent_scaling(h, True)
         assert sg.query_plan(h, 0, 0, 128, 128, 300, sg.fp16)[:2] == (1, 1)
+        sg.force_plan(h, 11, 4)
+        assert sg.query_plan(h, 0, 0, 16384, 256, 16384, sg.fp16) == (11, 4, FORCED)
+        assert sg.query_plan(h, 0, 0, 16384, 256, 16384, sg.tf32)[0] == 3
+        sg.set_exponent_scaling(h, False)
+        assert sg.query_plan(h, 0, 0, 16384, 256, 16384, sg.fp16)[0] == 3
+        sg.set_exponent_scaling(h, True)
+        assert sg.query_plan(h, 0, 0, 128, 128, 200, sg.fp16)[:2] == (1, 1)
     finally:
         sg.set_exponent_scaling(h, True)
         sg.force_plan(h, -1, 1)
@@ -87,12 +95,12 @@ def test_bad_plan_table_is_rejected(h, tmp_path):
     import shgemm_amd as sg
 
     bad = tmp_path / "bad.tsv"
-    bad.write_text("single fp16s N N N 10 10 10 11 1\n")  # variant 11 does not exist
+    bad.write_text("single fp16s N N N 10 10 10 12 1\n")  # variant 12 does not exist
     with pytest.raises(sg.ShgemmError) as ei:
         sg.load_plans(h, bad)
     assert ei.value.status == sg.STATUS_EINVAL and "bad.tsv:1" in str(ei.value)
     with pytest.raises(sg.ShgemmError):
-        sg.force_plan(h, 11, 1)
+        sg.force_plan(h, 12, 1)
 
 
 def test_table_plan_runs_named_kernel(h, tmp_path):
@@ -194,7 +202,7 @@ def _growing_rows(rng, m, k):
 
 
 @pytest.mark.parametrize("op_a,op_c", [(0, 0), (1, 1), (0, 1)])
-@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (2, 3), (3, 2), (4, 4), (10, 2)])
+@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (2, 3), (3, 2), (4, 4), (10, 2), (11, 3)])
 def test_running_exponents_follow_growing_rows(h, op_a, op_c, plan, slices):
     """The kernels with running row exponents rescale their accumulators when
     a row's maximum grows along k: rows growing, shrinking or starting with
@@ -231,13 +239,13 @@ def test_running_exponents_follow_growing_rows(h, op_a, op_c, plan, slices):
     assert np.all(r <= 2.0 * r_s + 2.0 ** -22), (plan, slices, float(r.max()), float(r_s.max()))
 
 
-KTILE = {0: 64, 1: 32, 2: 32, 3: 32, 4: 64, 10: 64}  # k-tile depth of each running-exponent plan's kernel
+KTILE = {0: 64, 1: 32, 2: 32, 3: 32, 4: 64, 10: 64, 11: 32}  # k-tile depth of each running-exponent plan's kernel
 
 
 @pytest.mark.parametrize("op_a,op_c", [(0, 0), (1, 1)])
 @pytest.mark.parametrize("plan,slices,k,step,width", [(0, 1, 320, 3, 32), (1, 1, 320, 3, 32), (2, 2, 1024, 4, 128),
                                                        (3, 2, 1024, 4, 128), (4, 2, 1024, 4, 128),
-                                                       (10, 2, 1024, 4, 128)])
+                                                       (10, 2, 1024, 4, 128), (11, 2, 1024, 4, 128)])
 def test_running_exponents_match_the_oracle(h, op_a, op_c, plan, slices, k, step, width):
     """Balanced growing rows (oracle.gen_growing_rows): the running-exponent kernels
     agree with the oracle's running-exponent emulation (k-tiles of their
@@ -280,7 +288,7 @@ def test_running_exponents_match_the_oracle(h, op_a, op_c, plan, slices, k, step
     assert oracle.rel_fro_error(got, ref) <= 2.0 * err_s
 
 
-@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (3, 2), (5, 1), (8, 1), (9, 1), (10, 2)])
+@pytest.mark.parametrize("plan,slices", [(0, 1), (1, 1), (3, 2), (5, 1), (8, 1), (9, 1), (10, 2), (11, 2)])
 def test_non_finite_inputs_stay_in_their_rows_and_columns(h, plan, slices):
     """A NaN or an infinity in A makes its row of C non-finite and one in B
     its column (the FP16 residual of an infinity is NaN, as in the

@@ -20,12 +20,13 @@ def test_parses_good_table(tmp_path):
                  "single fp16s N N N 10 10 10 0 1 16.5\n"
                  "single bf16 T N T 7 7 20 2 256\n"
                  "single fp16s N N N 7 7 20 10 256\n"
+                 "single fp16s N N N 14 8 14 11 4\n"
                  "grouped fp16 N N N 11 7 12 1 2 3.25\n")
-    assert sg.check_plan_file(f) == 4
+    assert sg.check_plan_file(f) == 5
 
 
 @pytest.mark.parametrize("line,why", [
-    ("single fp16s N N N 10 10 10 11 1", "variant 11"),
+    ("single fp16s N N N 10 10 10 12 1", "variant 12"),
     ("grouped fp16s N N N 10 10 10 4 1", "class 4"),
     ("single fp16x N N N 10 10 10 0 1", "mode"),
     ("single fp16s N X N 10 10 10 0 1", "op"),

@@ -47,6 +47,7 @@ CASES = [
     (3, 16, FP16, (640, 384, 1 << 16), (256, 128, 1 << 14)),     # large-tile split-K
     (2, 32, TF32, (128, 128, 1 << 18), (96, 64, 50000)),         # 128x128 split-K, BF16 split
     (10, 64, FP16, (128, 128, 1 << 18), (96, 64, 50000)),        # deep 128x128x64 split-K
+    (11, 4, FP16, (4096, 256, 8192), (1000, 200, 3000)),         # large tile of 64x128 wavefronts, split-K
     (5, 1, FP16, (2048, 1536, 4096), (1000, 700, 1500)),         # 256x256 tile, row pass
     (8, 1, FP16, (2048, 2048, 4096), (768, 512, 2048)),          # pre-split plan -> 256x256 tile
     (-1, 1, FP16, (4096, 4096, 4096), (1024, 1024, 1024)),       # the table's plans

@@ -17,6 +17,11 @@
     v_fma_mix forms) equals the oracle's split_f16_same_scale of x * s bit
     for bit, over 2^22 values: wide exponents, FP16's top range, rounding
     ties and FP16 subnormals, at power-of-two scales 2^-10 .. 2^9.
+  * The reference's remedy on gfx950: the hi product of every MFMA added
+    outside the chain has 1.3-2x less error than two MFMA chains; the
+    library's k-sliced plans for long k match or beat it.
+  * The device's streaming HBM read rate (the attainable bandwidth beside
+    the 8 TB/s peak).
 """
 from __future__ import annotations
 
@@ -45,6 +50,10 @@ def tk():
     L.tk_mfma.restype = ctypes.c_int
     L.tk_split_pair.argtypes = [ctypes.c_int, vp, vp, vp, vp]
     L.tk_split_pair.restype = ctypes.c_int
+    L.tk_chain_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+    L.tk_chain_gemm.restype = ctypes.c_int
+    L.tk_read_bw.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.tk_read_bw.restype = ctypes.c_int
     return L
 
 
@@ -240,3 +249,68 @@ def test_long_mfma_chain_vs_round_to_nearest_fp32(tk, k_log):
     assert errs["chain prescaled"] <= 2.0 * err_sg and errs["chain classic"] <= 2.0 * err_sg, (errs, err_sg)
     assert errs["chain prescaled"] > 2.0 * err_rn          # the chain's accumulation loses bits
     assert errs["64 slices"] <= 2.0 * err_rn and errs["library plan"] <= 2.0 * err_rn, (errs, err_rn)
+
+
+@pytest.mark.parametrize("k_log", [14, 16, 18])
+def test_hi_outside_the_chain_vs_two_chains(tk, k_log):
+    """The reference adds each MMA's hi product to its accumulator with an
+    FP32 add outside the MMA chain (src/wmmae_shgemm.hpp:86-98).  The same
+    split GEMM on gfx950 MFMA (m = n = 256, positive inputs), accumulated as
+    two MFMA chains (the library's classic mode), with the hi product outside
+    the chain, and with both outside: the chains lose bits against the
+    reference's remedy (1.3x its error or more, growing with k), the
+    residual's chain costs nothing (its products are 2^-11 of the hi ones),
+    and the library's own plan for these shapes, k-slices summed in FP32
+    outside the chain, is at least as accurate as the remedy from k = 2^16.
+    Both stay above the blocked round-to-nearest host GEMM, whose sum over k
+    is two-level."""
+    import torch
+
+    import shgemm_amd as sg
+
+    m = n = 256
+    k = 1 << k_log
+    rng = np.random.default_rng(100 + k_log)
+    A = rng.uniform(0.0, 1.0, (m, k)).astype(np.float32)        # row-major m x k
+    B16 = rng.uniform(0.0, 1.0, (n, k)).astype(np.float16)      # column j of B in row j
+    ref = A.astype(np.float64) @ B16.astype(np.float64).T
+    A_cm = np.asfortranarray(A).ravel(order="F")                # column-major m x k
+    host = oracle.sgemm_nn_packed(m, n, k, 1.0, A_cm, m, np.ascontiguousarray(B16.astype(np.float32)).ravel(), k,
+                                  0.0, None, m)
+    err_rn = oracle.rel_fro_error(host.reshape(n, m).T, ref)
+    errs = {}
+    for mode, name in ((0, "two chains"), (1, "hi outside"), (2, "both outside")):
+        C = np.empty((m, n), np.float32)
+        rc = tk.tk_chain_gemm(mode, m, n, k, _p(A), _p(B16.view(np.uint16)), _p(C))
+        assert rc == 0, f"tk_chain_gemm failed with HIP error {rc}"
+        errs[name] = oracle.rel_fro_error(C, ref)
+    h = sg.create()
+    try:
+        a, b = torch.from_numpy(A_cm).cuda(), torch.from_numpy(B16.ravel()).cuda()
+        c = torch.empty(m * n, device="cuda")
+        plan = sg.query_plan(h, 0, 0, m, n, k, sg.fp16)[:2]
+        sg.shgemm(h, 0, 0, m, n, k, 1.0, a, m, b, k, 0.0, c, m, sg.fp16)
+        errs["library plan"] = oracle.rel_fro_error(c.cpu().numpy().reshape(n, m).T, ref)
+    finally:
+        sg.destroy(h)
+    print(f"k=2^{k_log}: blocked round-to-nearest host FP32 {err_rn:.3e}, "
+          + ", ".join(f"{key} {v:.3e}" for key, v in errs.items()) + f" (plan {plan})", flush=True)
+    assert errs["two chains"] > 1.3 * errs["hi outside"], errs
+    assert abs(errs["both outside"] - errs["hi outside"]) <= 0.05 * errs["hi outside"], errs
+    if k_log >= 16:
+        assert errs["library plan"] <= errs["hi outside"], errs
+
+
+def test_hbm_read_ceiling(tk):
+    """The streaming read rate of this device (1 GiB, 16-byte loads, one
+    contiguous chunk per workgroup): the attainable figure beside the 8 TB/s
+    peak for the bandwidth-bound kernels."""
+    rates = {}
+    for grid in (2048, 4096, 8192):
+        ms = ctypes.c_float()
+        rc = tk.tk_read_bw(1 << 30, grid, 20, ctypes.byref(ms))
+        assert rc == 0, f"tk_read_bw failed with HIP error {rc}"
+        rates[grid] = (1 << 30) / (ms.value * 1e-3) / 1e12
+    print("HBM streaming read, 1 GiB: " + ", ".join(f"{g} workgroups {r:.2f} TB/s" for g, r in rates.items()),
+          flush=True)
+    assert 3.0 < max(rates.values()) < 8.0, rates

@@ -42,6 +42,21 @@ enum kernel_level {
   num_levels
 };
 struct handle_state;  // kernel table, workspace, device properties
+
+// reference shgemm.hpp:9-28: the function type and launch descriptor of the
+// reference's kernels.  Declared so that code naming them still compiles;
+// this library's kernels are described behind shgemmHandle_t::state and never
+// take this form.
+typedef void (*kernel_func_t)(const unsigned, const unsigned, const unsigned, const float, const float* const,
+                              const unsigned, const __half* const, const unsigned, const float, float* const,
+                              const unsigned, float* const, const unsigned);
+struct kernel {
+  unsigned smem_m, smem_n, smem_k;
+  unsigned num_blocks_filling = 0xffffffffu;
+  unsigned block_size;
+  unsigned smem_size;
+  kernel_func_t func;
+};
 }  // namespace detail
 
 // reference shgemm.hpp:36-39

@@ -20,6 +20,11 @@
 
 namespace {
 
+// The reference header's kernel descriptor types are part of its interface.
+static_assert(sizeof(mtk::shgemm::detail::kernel) >= sizeof(mtk::shgemm::detail::kernel_func_t),
+              "detail::kernel holds a kernel_func_t");
+static_assert(mtk::shgemm::detail::num_levels == 2, "two kernel levels");
+
 #define CHECK_HIP(x)                                                              \
   do {                                                                            \
     const hipError_t e_ = (x);                                                    \

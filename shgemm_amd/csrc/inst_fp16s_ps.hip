@@ -1,8 +1,8 @@
-// Pre-split prescaled fp16 kernels (A split once per call, op_b = n): C op_n
-// and op_t.  The tile is the prescaled P2 tile.
+// Pre-split prescaled fp16 kernels (A split once per call, A and B packed in
+// tile order): C op_n and op_t, on the 256 x 256 tile of eight wavefronts.
 #include "register_shgemm.hpp"
 namespace shgemm_amd {
-using CfgPS = ModeCfg<MODE_FP16S>::P2;
+using CfgPS = ModeCfg<MODE_FP16S>::PS;
 template <int OPC>
 static void put_ps(KernelTable& t) {
   static_assert(CfgPS::lds_bytes<MODE_FP16S>() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");

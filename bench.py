@@ -42,7 +42,7 @@ def parse():
                          "energy: W and J per GEMM from the card's energy counter")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--sample-rows", type=int, default=256, help="rows of C checked against FP64")
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU-baseline sample")
@@ -427,17 +427,27 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
-def _time_ms(fn, iters, stream):
+def _time_ms(fn, iters, stream, warm_ms=100.0):
     """Median per-call milliseconds over 3 batches of `iters` calls (HIP events
-    on `stream`), after `iters` untimed calls that bring the GPU out of the
-    idle it falls into while the host computes the previous line's errors."""
+    on `stream`), after untimed batches of `iters` calls until at least
+    `warm_ms` of GPU time has passed: the GPU idles while the host computes
+    the previous line's errors, and its clock takes tens of milliseconds of
+    work to come back (a 0.34 ms call measured 30% slower after 10 calls)."""
     import statistics
 
     import torch
 
-    for _ in range(iters):
-        fn()
-    torch.cuda.synchronize()
+    warm = 0.0
+    while True:
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(iters):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        warm += s.elapsed_time(e)
+        if warm >= warm_ms:
+            break
     out = []
     for _ in range(3):
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

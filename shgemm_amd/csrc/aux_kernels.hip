@@ -480,9 +480,10 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
 // and 32-deep k-tile, 16 KiB holding the tile's LDS image (slot
 // lds_slot_h<4>(j, c) = column j's k-values 8c..8c+7, FP16 bits unchanged),
 // columns past n zero.  B is k x n with op_b applied: op_n stores element
-// (kk, j) at b[kk + j * ldb] (k contiguous), op_t at b[j + kk * ldb].
+// (kk, j) at b[kk + j * ldb] (k contiguous), op_t at b[j + kk * ldb].  Loads
+// are 16 bytes at any ldb and base alignment.
 template <bool KCONT>
-__global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict__ b, int64_t ldb, int n, int k, int vec,
+__global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict__ b, int64_t ldb, int n, int k,
                                                      uint4* __restrict__ out) {
   __shared__ uint16_t t[KCONT ? 1 : kPackK][kPackRows + 8];
   const int nb = (n + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
@@ -495,8 +496,8 @@ __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict_
       const int i = tid + 256 * q;
       const int kk = i / (kPackRows / 8), jj = 8 * (i % (kPackRows / 8));
       const uint16_t* src = b + (int64_t)(j0 + jj) + (int64_t)(k0 + kk) * ldb;
-      if (vec && j0 + jj + 8 <= n) {
-        const uint4 w = *reinterpret_cast<const uint4*>(src);
+      if (j0 + jj + 8 <= n) {
+        const uint4 w = load_unaligned<uint4>(src);
         *reinterpret_cast<uint4*>(&t[kk][jj]) = w;
       } else {
 #pragma unroll
@@ -513,16 +514,7 @@ __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict_
     uint4 w;
     if constexpr (KCONT) {
       const uint16_t* src = b + (int64_t)(j0 + j) * ldb + k0 + 8 * c;
-      if (j0 + j >= n) {
-        w = uint4{0u, 0u, 0u, 0u};
-      } else if (vec) {
-        w = *reinterpret_cast<const uint4*>(src);
-      } else {
-        uint32_t h[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = (uint32_t)src[2 * e] | ((uint32_t)src[2 * e + 1] << 16);
-        w = uint4{h[0], h[1], h[2], h[3]};
-      }
+      w = j0 + j >= n ? uint4{0u, 0u, 0u, 0u} : load_unaligned<uint4>(src);
     } else {
       uint32_t h[4];
 #pragma unroll
@@ -542,11 +534,10 @@ hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k,
   if (k % kPackK != 0) return hipErrorInvalidValue;
   const int64_t blocks = (int64_t)((n + kPackRows - 1) / kPackRows) * (k / kPackK);
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
-  const int vec = (reinterpret_cast<uintptr_t>(b) % 16 == 0) && (ldb % 8 == 0);
   if (op_b == OP_N)
-    hipLaunchKernelGGL(pack_b_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, vec, out);
+    hipLaunchKernelGGL(pack_b_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, out);
   else
-    hipLaunchKernelGGL(pack_b_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, vec, out);
+    hipLaunchKernelGGL(pack_b_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, out);
   return hipGetLastError();
 }
 

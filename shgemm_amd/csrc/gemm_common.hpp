@@ -42,7 +42,7 @@ struct GemmParams {
   int k_slice;         // k extent per slice (multiple of the tile depth)
   int tiles_m, tiles_n;
   int group_m;         // tile rows per L2 group
-  int vec_a, vec_b;    // 1 when base pointers and ld allow vector loads
+  int vec_a, vec_b;    // 1 when base pointers and ld allow aligned 16-byte loads (the direct-to-LDS plans)
 };
 
 // Workgroup id -> output tile.  Consecutive workgroup ids are dispatched
@@ -196,6 +196,19 @@ __device__ __forceinline__ const T* uniform_ptr(const T* p) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
   return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
+// A T-sized value from an address aligned only to its element type (a row of
+// FP16 B at an odd ld, op_t A at an ld that is not a multiple of 4).  The
+// copy compiles to one global load of sizeof(T) bytes: gfx9 global memory
+// runs in unaligned mode under ROCm, so the hardware splits the access where
+// it crosses a cache line, and the load costs about as much as an aligned one
+// (eight 2-byte loads per 16 bytes cost twice the whole GEMM at k = 16001).
+template <class T, class E>
+__device__ __forceinline__ T load_unaligned(const E* p) {
+  T v;
+  __builtin_memcpy(&v, p, sizeof(T));
+  return v;
 }
 
 __device__ __forceinline__ _Float16 bits_to_f16(uint16_t u) {

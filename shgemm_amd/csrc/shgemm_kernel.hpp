@@ -90,8 +90,8 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
       M::item(tid + NT * p, r, c);
       if constexpr (KCONT) {
         const float* ptr = base + (int64_t)(r0 + r) * ld + (k0 + 8 * c);
-        const f32x4 x = *reinterpret_cast<const f32x4*>(ptr);
-        const f32x4 y = *reinterpret_cast<const f32x4*>(ptr + 4);
+        const f32x4 x = load_unaligned<f32x4>(ptr);
+        const f32x4 y = load_unaligned<f32x4>(ptr + 4);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           v[p][e] = x[e];
@@ -250,33 +250,11 @@ struct TileF16 {
       int r, c;
       item(tid + NT * p, r, c);
       if constexpr (KCONT) {
-        vk[p] = *reinterpret_cast<const u32x4*>(base + (int64_t)(r0 + r) * ld + (k0 + 8 * c));
+        vk[p] = load_unaligned<u32x4>(base + (int64_t)(r0 + r) * ld + (k0 + 8 * c));
       } else {
         const uint16_t* ptr = base + (r0 + r) + (int64_t)(k0 + 8 * c) * ld;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) vr[p][e] = *reinterpret_cast<const uint32_t*>(ptr + (int64_t)e * ld);
-      }
-    }
-  }
-
-  // Full tile, any ld and base alignment: 2-byte loads, no bounds checks.
-  __device__ __forceinline__ void load_fast_u16(const uint16_t* __restrict__ base, int64_t ld, int r0, int k0,
-                                                int tid) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int r, c;
-      item(tid + NT * p, r, c);
-      if constexpr (KCONT) {
-        const uint16_t* ptr = base + (int64_t)(r0 + r) * ld + (k0 + 8 * c);
-        uint32_t w[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = (uint32_t)ptr[2 * q] | ((uint32_t)ptr[2 * q + 1] << 16);
-        vk[p] = u32x4{w[0], w[1], w[2], w[3]};
-      } else {
-        const uint16_t* ptr = base + (r0 + r) + (int64_t)(k0 + 8 * c) * ld;
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          vr[p][e] = (uint32_t)ptr[(int64_t)e * ld] | ((uint32_t)ptr[(int64_t)e * ld + 1] << 16);
+        for (int e = 0; e < 8; ++e) vr[p][e] = load_unaligned<uint32_t>(ptr + (int64_t)e * ld);
       }
     }
   }
@@ -728,10 +706,10 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const float* __restrict__ A = p.a;
   const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
-  // unchecked loads: op_n A uses 4-byte loads (any lda); the 16-byte forms
-  // need vec_a / vec_b
-  const bool a_full = (OPA == OP_N || p.vec_a) && (m0 + BM <= p.m);
-  const bool b_full = p.vec_b && (n0 + BN <= p.n);
+  // unchecked loads: op_n A uses 4-byte loads, the other forms 16- or 4-byte
+  // loads at any ld and base alignment (load_unaligned)
+  const bool a_full = m0 + BM <= p.m;
+  const bool b_full = n0 + BN <= p.n;
 
   // RUN: running row exponents kept by the tile (no separate pass over A).
   // Every thread keeps the current exponent of its items' rows (ecur, the
@@ -938,12 +916,7 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
         fast_step(P1{}, bload);
       }
     };
-    if (a_full && b_full) {
-      fast_loop([&](int k1) { tb[0].load_fast(B, p.ldb, n0, k1, tid); });
-    } else if (a_full && n0 + BN <= p.n) {
-      // B rows whole but ldb or the base not 16-byte aligned
-      fast_loop([&](int k1) { tb[0].load_fast_u16(B, p.ldb, n0, k1, tid); });
-    }
+    if (a_full && b_full) fast_loop([&](int k1) { tb[0].load_fast(B, p.ldb, n0, k1, tid); });
     for (; kt < ktiles; ++kt) {
       const int cur = kt & 1;
       if (kt + 1 < ktiles) gload(S0{}, kt + 1);

@@ -172,6 +172,19 @@ __device__ __forceinline__ int exponent_of_max(uint32_t b) {
   return e;
 }
 
+// The smallest |a| bit pattern b with exponent_of_max(b) > e, for
+// -126 <= e <= 113: 2^(e + 15), or the infinity pattern for e = 113 (no
+// finite value exceeds it).  For every finite b, b >= rise_threshold(e)
+// exactly when exponent_of_max(b) > e; an infinity or NaN pattern is always
+// at or above it.
+__device__ __forceinline__ uint32_t rise_threshold(int e) {
+  return e < 113 ? (uint32_t)(e + 142) << 23 : 0x7f800000u;
+}
+// The e of a rise_threshold(e).
+__device__ __forceinline__ int threshold_exponent(uint32_t t) {
+  return t < 0x7f800000u ? (int)(t >> 23) - 142 : 113;
+}
+
 // FP32 -> three BF16 parts a0 + a1 + a2 (each residual exact in FP32).
 __device__ __forceinline__ void split_bf16x3(float a, __bf16& a0, __bf16& a1, __bf16& a2) {
   a0 = (__bf16)a;

@@ -43,6 +43,8 @@ struct ItemMap {
   static constexpr int PER = NITEMS / NT;
   static_assert(NITEMS % NT == 0 && PER >= 1, "tile must split evenly over the workgroup");
   static_assert(KCONT || R % 64 == 0, "row-fastest map needs whole wavefronts per chunk");
+  // row-fastest with R dividing NT: every item of a thread lies in its row tid % R
+  static constexpr bool ONE_ROW = !KCONT && NT % R == 0;
   __device__ __forceinline__ static void item(int i, int& r, int& c) {
     if constexpr (KCONT) {
       r = i / KC;
@@ -72,14 +74,12 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
     }
   }
 
-  // |a| bits of the largest of item p's 8 values
-  __device__ __forceinline__ uint32_t item_max(int p) const {
-    uint32_t mx = 0;
+  // The largest |a| of item p's 8 values (a NaN is passed over unless all
+  // are NaN): v_max3 with |.| operand modifiers, four instructions.
+  __device__ __forceinline__ float item_absmax(int p) const {
+    float mx = __builtin_fabsf(v[p][0]);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const uint32_t b = __builtin_bit_cast(uint32_t, v[p][e]) & 0x7fffffffu;
-      mx = b > mx ? b : mx;
-    }
+    for (int e = 1; e < 8; ++e) mx = __builtin_fmaxf(mx, __builtin_fabsf(v[p][e]));
     return mx;
   }
 
@@ -712,13 +712,15 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   const bool b_full = n0 + BN <= p.n;
 
   // RUN: running row exponents kept by the tile (no separate pass over A).
-  // Every thread keeps the current exponent of its items' rows (ecur, the
-  // same in every thread holding a chunk of the row) and splits each k-tile
-  // with it.  A chunk whose maximum needs a larger exponent merges that
-  // maximum into rmax[t&1] and records t in flag[t&1] before the barrier
-  // that ends the k-tile's stores; after it the workgroup reads the flag
-  // (uniform) and, only when it is set, gives each row the exponent
-  // max(ecur, that of rmax), splits the k-tile again from the registers into
+  // Every thread keeps the current exponent e of its items' rows (the same
+  // in every thread holding a chunk of the row) as the threshold
+  // ethr = rise_threshold(e) and splits each k-tile with it.  A chunk whose
+  // largest |a| reaches the threshold (one compare: its exponent would be
+  // larger than e) merges that maximum into rmax[t&1] and records t in
+  // flag[t&1] before the barrier that ends the k-tile's stores; a thread
+  // whose items share one row tests their common maximum once.  After the
+  // barrier the workgroup reads the flag (uniform) and, only when it is set,
+  // gives each row the exponent max(e, that of rmax), splits the k-tile again from the registers into
   // the same stage, records the factor 2^(old - new) and waits at one more
   // barrier.  Before the k-tile's MFMAs the accumulators of a tile that rose
   // are scaled by their rows' factors (exact powers of two).  Rows start at
@@ -730,7 +732,7 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
   using RS = RunState<BM>;
   using MI = ItemMap<OPA == OP_T, BM, BK, NT>;
   RS* rs = nullptr;
-  int ecur[MI::PER];
+  uint32_t ethr[MI::PER];
   if constexpr (RUN) {
     rs = reinterpret_cast<RS*>(lds + 2 * STAGE);
     for (int r = tid; r < BM; r += NT) {
@@ -741,7 +743,7 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     if (tid < 2) rs->flag[tid] = -1;
 #pragma unroll
     for (int q = 0; q < MI::PER; ++q) {
-      ecur[q] = kFloor;
+      ethr[q] = rise_threshold(kFloor);
 #pragma unroll
       for (int i = 0; i < PF; ++i) ta[i].scale[q] = __builtin_ldexpf(1.0f, -kFloor);
     }
@@ -783,21 +785,36 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       tb[set].store_bf16(st + NA * A_CH, st + NA * A_CH + B_CH, tid);
     }
   };
-  // RUN, before k-tile t (in register set `set_c`) is stored: flag the
-  // chunks that need a larger exponent (par_c == t & 1, an int or an
-  // integral_constant)
+  // RUN, after k-tile t (in register set `set_c`) is stored at the current
+  // exponents and before the barrier that ends the stores: flag the chunks
+  // that need a larger exponent (par_c == t & 1, an int or an
+  // integral_constant).  A k-tile that rises is split and stored again, so
+  // the store need not wait for the check (nor the split of the first
+  // loaded values for the last loads).
   auto run_check = [&](auto set_c, int t, auto par_c) {
     constexpr int set = decltype(set_c)::value;
     const int par = static_cast<int>(par_c);
+    if constexpr (MI::ONE_ROW) {
+      float mx = ta[set].item_absmax(0);
 #pragma unroll
-    for (int q = 0; q < MI::PER; ++q) {
-      const uint32_t b = ta[set].item_max(q);
-      const int et = b == 0u ? kFloor : exponent_of_max(b);
-      if (et > ecur[q]) {
+      for (int q = 1; q < MI::PER; ++q) mx = __builtin_fmaxf(mx, ta[set].item_absmax(q));
+      const uint32_t b = __builtin_bit_cast(uint32_t, mx);
+      if (b >= ethr[0]) {
         int r, c;
-        MI::item(tid + NT * q, r, c);
+        MI::item(tid, r, c);
         atomicMax(&rs->rmax[par][r], b);
         rs->flag[par] = t;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < MI::PER; ++q) {
+        const uint32_t b = __builtin_bit_cast(uint32_t, ta[set].item_absmax(q));
+        if (b >= ethr[q]) {
+          int r, c;
+          MI::item(tid + NT * q, r, c);
+          atomicMax(&rs->rmax[par][r], b);
+          rs->flag[par] = t;
+        }
       }
     }
   };
@@ -813,13 +830,14 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       int r, c;
       MI::item(tid + NT * q, r, c);
       const uint32_t b = rs->rmax[par][r];
+      const int e0 = threshold_exponent(ethr[q]);
       const int et = b == 0u ? kFloor : exponent_of_max(b);
-      const int en = et > ecur[q] ? et : ecur[q];
+      const int en = et > e0 ? et : e0;
       if (c == 0) {
-        rs->fac[r] = __builtin_ldexpf(1.0f, ecur[q] - en);
+        rs->fac[r] = __builtin_ldexpf(1.0f, e0 - en);
         rs->erow[r] = en;
       }
-      ecur[q] = en;
+      ethr[q] = rise_threshold(en);
 #pragma unroll
       for (int i = 0; i < PF; ++i) ta[i].scale[q] = __builtin_ldexpf(1.0f, -en);
     }
@@ -874,8 +892,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     bool rose = false;  // RUN: some row's exponent rose at the next k-tile to multiply
     if (ktiles > 0) {
       gload(S0{}, 0);
-      if constexpr (RUN) run_check(S0{}, 0, P0{});
       sstore(S0{}, 0);
+      if constexpr (RUN) run_check(S0{}, 0, P0{});
     }
     __syncthreads();
     if constexpr (RUN) {
@@ -901,11 +919,9 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
         if (rose) run_rescale();
       }
       compute(lds + cur * STAGE);
-      if constexpr (RUN) {
-        __builtin_amdgcn_sched_barrier(0);  // the exponent checks wait for the loads after the MFMAs
-        run_check(S0{}, kt + 1, NX{});
-      }
+      if constexpr (RUN) __builtin_amdgcn_sched_barrier(0);  // the split waits for the loads after the MFMAs
       sstore(S0{}, cur ^ 1);
+      if constexpr (RUN) run_check(S0{}, kt + 1, NX{});
       __syncthreads();
       if constexpr (RUN) rose = run_settle(S0{}, kt + 1, NX{}, cur ^ 1);
       ++kt;
@@ -925,8 +941,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       }
       compute(lds + cur * STAGE);
       if (kt + 1 < ktiles) {
-        if constexpr (RUN) run_check(S0{}, kt + 1, cur ^ 1);
         sstore(S0{}, cur ^ 1);
+        if constexpr (RUN) run_check(S0{}, kt + 1, cur ^ 1);
       }
       __syncthreads();
       if constexpr (RUN) {
@@ -940,8 +956,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
     if (ktiles > 0) gload(S0{}, 0);
     if (ktiles > 1) gload(S1{}, 1);
     if (ktiles > 0) {
-      if constexpr (RUN) run_check(S0{}, 0, Q0{});
       sstore(S0{}, 0);
+      if constexpr (RUN) run_check(S0{}, 0, Q0{});
     }
     __syncthreads();
     if constexpr (RUN) {
@@ -1026,11 +1042,9 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
           if (rose) run_rescale();
         }
         compute(lds);
-        if constexpr (RUN) {
-          __builtin_amdgcn_sched_barrier(0);
-          run_check(S1{}, kt + 1, Q1{});
-        }
+        if constexpr (RUN) __builtin_amdgcn_sched_barrier(0);
         sstore(S1{}, 1);
+        if constexpr (RUN) run_check(S1{}, kt + 1, Q1{});
         __syncthreads();
         if constexpr (RUN) rose = run_settle(S1{}, kt + 1, Q1{}, 1);
         ta[1].load_fast(A, p.lda, m0, k2 + BK, tid);
@@ -1040,11 +1054,9 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
           if (rose) run_rescale();
         }
         compute(lds + STAGE);
-        if constexpr (RUN) {
-          __builtin_amdgcn_sched_barrier(0);
-          run_check(S0{}, kt + 2, Q0{});
-        }
+        if constexpr (RUN) __builtin_amdgcn_sched_barrier(0);
         sstore(S0{}, 0);
+        if constexpr (RUN) run_check(S0{}, kt + 2, Q0{});
         __syncthreads();
         if constexpr (RUN) rose = run_settle(S0{}, kt + 2, Q0{}, 0);
       }
@@ -1056,8 +1068,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       }
       compute(lds);
       if (kt + 1 < ktiles) {
-        if constexpr (RUN) run_check(S1{}, kt + 1, Q1{});
         sstore(S1{}, 1);
+        if constexpr (RUN) run_check(S1{}, kt + 1, Q1{});
       }
       __syncthreads();
       if constexpr (RUN) {
@@ -1070,8 +1082,8 @@ __device__ __forceinline__ void shgemm_tile(const GemmParams& p, int tm, int tn,
       }
       compute(lds + STAGE);
       if (kt + 2 < ktiles) {
-        if constexpr (RUN) run_check(S0{}, kt + 2, Q0{});
         sstore(S0{}, 0);
+        if constexpr (RUN) run_check(S0{}, kt + 2, Q0{});
       }
       __syncthreads();
       if constexpr (RUN) {

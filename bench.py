@@ -30,6 +30,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_FP16_DENSE = 2516.6   # TFLOP/s, MI355X FP16 dense MFMA
 PEAK_FP32_MATRIX = 157.3   # TFLOP/s, MI355X FP32 matrix
 PEAK_HBM = 8.0             # TB/s
+NOMINAL_CLOCK_MHZ = 2400.0  # GFX clock of the published MFMA peaks (256 CUs x 4 SIMDs x 1024 FP16 FLOP/clock)
 P99_BOUND_MS = 10.0        # stated sketch-server latency bound (p99, ms) up to the reported rate
 
 
@@ -48,6 +49,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="square: skip the short serving run")
+    ap.add_argument("--no-power", action="store_true", help="square: skip the 2 s clock and power window")
+    ap.add_argument("--sketch-data", action="store_true",
+                    help="energy: only the sketch shape, on A = U[-1,1) and on A = 0 (the data dependence of the clock)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="square: offered rate of the serving run")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="square: arrival window of the serving run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -62,6 +66,31 @@ def parse():
                     help="serve: requests admitted together to one lane run as one grouped launch (up to this many)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget")
     return ap.parse_args()
+
+
+def power_window(fn, seconds=2.0):
+    """GFX clock, socket power and energy per call while `fn` repeats for
+    about `seconds` (shgemm_amd.energy: the card's energy accumulator around
+    the window, the metrics table sampled every 50 ms); None when ROCm SMI is
+    not importable.  The metrics table refreshes too slowly for a window of a
+    few calls, so this is a window of its own after the timed steps."""
+    try:
+        import torch
+
+        from shgemm_amd.energy import EnergyMeter, measure
+
+        meter = EnergyMeter()
+    except Exception:
+        return None
+    try:
+        r = measure(meter, fn, seconds, torch.cuda.synchronize)
+    finally:
+        meter.close()
+    return {"median_gfxclk_mhz": r["median_gfxclk_mhz"], "median_socket_power_w": r["median_socket_power_w"],
+            "avg_power_w": round(r["avg_power_w"], 1), "energy_per_call_j": round(r["energy_per_call_j"], 4),
+            "calls": r["count"], "window_s": round(r["window_s"], 2),
+            "source": "ROCm SMI, a window of the same call repeated after the timed steps: the card's energy "
+                      "accumulator around it, GFX clock and socket power medians of the metrics table"}
 
 
 def timed_launches(fn, steps, warmup, stream, after_warmup=None):
@@ -662,6 +691,32 @@ def bench_energy(args):
     sg.set_cuda_stream(h, stream)
     g = torch.Generator(device=dev)
     g.manual_seed(args.seed)
+    if args.sketch_data:
+        # The sketch shape's fp16 call on A = U[-1,1) and on A = 0 (same plan,
+        # same instructions, operands that toggle nothing): under the power cap
+        # the clock, and with it the call's time, follows the data.
+        m, n, k = 16384, 256, 16384
+        Bs = (torch.randn(k * n, device=dev, generator=g) / 16.0).half()
+        Cs = torch.empty(m * n, device=dev)
+        for name in ("uniform[-1,1)", "zeros", "uniform[-1,1)"):
+            As = (torch.rand(m * k, device=dev, generator=g) * 2 - 1) if name != "zeros" else torch.zeros(m * k, device=dev)
+            fn = lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, As, m, Bs, k, 0.0, Cs, m, sg.fp16)
+            r = measure(meter, fn, args.seconds, torch.cuda.synchronize)
+            for _ in range(20):
+                fn()
+            sg.enable_kernel_timing(h, 50)
+            _, _, call_ms = timed_launches(fn, 50, 0, stream)
+            nl, kt = sg.kernel_timing(h)
+            sg.enable_kernel_timing(h, 0)
+            print(json.dumps({"config": "sketch shgemm nn fp16", "m": m, "n": n, "k": k, "A": name,
+                              "call_us": round(call_ms * 1e3, 1), "gemm_kernel_us": round(kt / max(nl, 1) * 1e3, 1),
+                              "avg_power_w": round(r["avg_power_w"], 1), "energy_per_call_j": round(r["energy_per_call_j"], 4),
+                              "median_gfxclk_mhz": r["median_gfxclk_mhz"],
+                              "median_socket_power_w": r["median_socket_power_w"]}), flush=True)
+            del As
+        meter.close()
+        sg.destroy(h)
+        return
     top = 1 << 14
     A = torch.rand(top * top, device=dev, generator=g)
     B = torch.rand(top * top, device=dev, generator=g).half()
@@ -751,6 +806,10 @@ def main():
     region_ms, wall_ms, _ = timed_launches(step, args.steps, args.warmup, stream,
                                            after_warmup=lambda: sg.enable_kernel_timing(h, args.steps))
     kern_ms = kernel_ms()
+    # The card's GFX clock and power on this call (ROCm SMI, when importable):
+    # under the card's power cap the clock, not the nominal 2.4 GHz behind the
+    # MFMA peak, bounds the kernel.
+    clock = None if args.no_power else power_window(step)
     s_steps = max(2, args.steps // 2)
     sstep = lambda: sg.sgemm_fp32(h, sg.op_n, sg.op_n, m, n, k, 1.0, A, m, B32, k, 0.0, Cs, m)
     s_region_ms, _, _ = timed_launches(sstep, s_steps, 1, stream, after_warmup=lambda: sg.enable_kernel_timing(h, s_steps))
@@ -809,6 +868,7 @@ def main():
 
     achieved = flops / (kern_ms * 1e-3) / 1e12
     peak_eff = PEAK_FP16_DENSE / 2.0
+    clk = clock.get("median_gfxclk_mhz") if clock else None
     traffic, traffic_src = None, None
     tf = ROOT / "profiles" / "traffic_shgemm_16384.json"
     if tf.exists():
@@ -848,17 +908,22 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak_eff, 1), "unit": "TFLOP/s",
                      "frac": round(achieved / peak_eff, 4),
                      "frac_of_fp16_dense_peak": round(achieved / PEAK_FP16_DENSE, 4),
+                     "frac_at_measured_clock": (round(achieved / (peak_eff * clk / NOMINAL_CLOCK_MHZ), 4)
+                                                if isinstance(clk, (int, float)) and clk > 0 else None),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "note": "achieved = 2mnk / mean duration of the main GEMM kernel alone (HIP events the library "
                              "records around it on the launch stream); left out of that duration and counted in "
                              "ms_per_step: the row-exponent pass over A and, on the pre-split plan, the pack "
                              "pass of A (split into FP16 hi/lo) and of B; peak = FP16 dense 2516.6 / 2, since "
                              "every algorithmic FMA costs two FP16 MFMA products (hi and lo); "
-                             "frac_of_fp16_dense_peak = achieved / 2516.6",
+                             "frac_of_fp16_dense_peak = achieved / 2516.6; frac_at_measured_clock = achieved / "
+                             "(peak x median GFX clock of the call's power window / 2400 MHz, the clock of the "
+                             "nominal peak)",
                      "kernel_ms": round(kern_ms, 4)},
         "cpu_baseline": cpu,
         "cpu_configs": cpu_cfg,
         "sketch_serving": serving,
+        "clock_and_power": clock,
         "wall_ms_timed_region": round(wall_ms, 3),
     }
     print(json.dumps(line), flush=True)

@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="square: skip the short serving run")
-    ap.add_argument("--no-power", action="store_true", help="square: skip the 2 s clock and power window")
+    ap.add_argument("--no-power", action="store_true",
+                    help="square, configs, grouped: skip the 2 s clock and power window after the timed calls")
     ap.add_argument("--sketch-data", action="store_true",
                     help="energy: only the sketch shape, on A = U[-1,1) and on A = 0 (the data dependence of the clock)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="square: offered rate of the serving run")
@@ -66,6 +67,14 @@ def parse():
                     help="serve: requests admitted together to one lane run as one grouped launch (up to this many)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget")
     return ap.parse_args()
+
+
+def _frac_at_clock(flops, byts, ms, power):
+    clk = power.get("median_gfxclk_mhz") if power else None
+    if not isinstance(clk, (int, float)) or clk <= 0:
+        return None
+    t_mfma = flops / (PEAK_FP16_DENSE / 2.0 * 1e12) * NOMINAL_CLOCK_MHZ / clk
+    return round(max(t_mfma, byts / (PEAK_HBM * 1e12)) / (ms * 1e-3), 4)
 
 
 def power_window(fn, seconds=2.0):
@@ -229,6 +238,7 @@ def bench_grouped(args):
     cls, slices, units = grp.plan()
     run_group = lambda: grp.run(stream)
     g_region, g_wall, g_ms = timed_launches(run_group, args.steps, args.warmup, stream)
+    g_power = None if args.no_power else power_window(run_group)
 
     def per_problem():
         for m, n, k, A, lda, B, ldb, C, ldc, al, be in probs:
@@ -311,8 +321,12 @@ def bench_grouped(args):
         "error": {"max_ratio_vs_sgemm": max(errs), "sampled_problems": len(errs), "rows_per_problem": 64},
         "roofline": {"bound": "mfma" if flops / (PEAK_FP16_DENSE / 2.0 * 1e12) > byts / (PEAK_HBM * 1e12) else "hbm",
                      "achieved": round(flops / (g_ms * 1e-3) / 1e12, 3), "peak": round(PEAK_FP16_DENSE / 2.0, 1),
-                     "unit": "TFLOP/s", "frac": round(t_roof * 1e3 / g_ms, 4), "traffic": None,
-                     "note": "frac = batch roofline time max(flops/1258 TF, bytes/8 TB/s) / measured graph time"},
+                     "unit": "TFLOP/s", "frac": round(t_roof * 1e3 / g_ms, 4),
+                     "frac_at_measured_clock": _frac_at_clock(flops, byts, g_ms, g_power),
+                     "traffic": None,
+                     "note": "frac = batch roofline time max(flops/1258 TF, bytes/8 TB/s) / measured graph time; "
+                             "frac_at_measured_clock scales the MFMA term to the GFX clock of the batch's power window"},
+        "clock_and_power": g_power,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -510,7 +524,7 @@ def bench_configs(args):
     g.manual_seed(args.seed)
     rng = np.random.default_rng(args.seed)
 
-    def emit(name, m, n, k, ms, err, err_s, extra):
+    def emit(name, m, n, k, ms, err, err_s, extra, power=None):
         flops = 2.0 * m * n * k
         byts = 4.0 * m * k + 2.0 * k * n + 4.0 * m * n
         t_mfma = flops / (PEAK_FP16_DENSE / 2.0 * 1e12) * (2.5 if extra.get("compute_type") == "tf32" else 1.0)
@@ -521,6 +535,12 @@ def bench_configs(args):
                 "bound": "mfma" if t_mfma > t_hbm else "hbm",
                 "roofline_frac": round(max(t_mfma, t_hbm) / (ms * 1e-3), 4),
                 "err": err, "sgemm_err": err_s, "err_ratio": err / err_s}
+        clk = power.get("median_gfxclk_mhz") if power else None
+        if isinstance(clk, (int, float)) and clk > 0:
+            # the MFMA term at the clock the card ran this call at (HBM's does not scale)
+            line["roofline_frac_at_measured_clock"] = round(
+                max(t_mfma * NOMINAL_CLOCK_MHZ / clk, t_hbm) / (ms * 1e-3), 4)
+            line["clock_and_power"] = power
         line.update(extra)
         print(json.dumps(line), flush=True)
 
@@ -535,6 +555,7 @@ def bench_configs(args):
     C = torch.empty(m * n, device=dev)
     Cs = torch.empty(m * n, device=dev)
     ms = _time_ms(lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16), 50, stream)
+    pw = None if args.no_power else power_window(lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16))
     sg.sgemm_fp32(h, 0, 0, m, n, k, 1.0, A, m, B.float(), k, 0.0, Cs, m)
     torch.cuda.synchronize()
     An = A.view(k, m).t().cpu().numpy()
@@ -542,7 +563,7 @@ def bench_configs(args):
     plan = sg.query_plan(h, 0, 0, m, n, k, sg.fp16)
     emit("cfg1 1024^3 nn fp16", m, n, k, ms, rows_err(An, Bn, C.view(n, m).t().cpu().numpy()),
          rows_err(An, Bn, Cs.view(n, m).t().cpu().numpy()),
-         {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[plan[0]], plan[1]]})
+         {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[plan[0]], plan[1]]}, pw)
     del A, B, C, Cs
 
     # ---- sketch (wide exponent) -----------------------------------------------
@@ -575,10 +596,11 @@ def bench_configs(args):
                 call = lambda: sg.hsgemm(h, 1, 1, l, m, k, 1.0, Om, k, A, m, 0.0, Y, l, ct)
                 q = sg.query_plan(h, 0, 0, m, l, k, ct, 1)
             ms = _time_ms(call, 10, stream)
+            pw = None if args.no_power or variant != "shgemm nn" else power_window(call)
             torch.cuda.synchronize()
             Yh = Y.view(m, l)[ridx] if variant == "hsgemm" else Y.view(l, m).t()[ridx]
             emit(f"sketch {variant} {ctn}", m, l, k, ms, rows_err(A_rows, Om64, Yh.cpu().numpy()), err_s,
-                 {"compute_type": ctn, "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "prescaled": ctn == "fp16"})
+                 {"compute_type": ctn, "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "prescaled": ctn == "fp16"}, pw)
     del A, At, Om, Om32, Y, Ys
     torch.cuda.empty_cache()
 
@@ -592,6 +614,8 @@ def bench_configs(args):
     for scaling in (True, False):
         sg.set_exponent_scaling(h, scaling)
         ms = _time_ms(lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16), 20, stream)
+        pw = None if args.no_power or not scaling else power_window(
+            lambda: sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16))
         runs = []
         for _ in range(10):
             sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, sg.fp16)
@@ -607,7 +631,7 @@ def bench_configs(args):
         q = sg.query_plan(h, 0, 0, m, n, k, sg.fp16)
         emit(f"cfg3 128x128x2^20 fp16 {'prescaled' if scaling else 'classic split'}", m, n, k, ms,
              rows_err(An, Bn, C.view(n, m).t().cpu().numpy()), err_s,
-             {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "bit_identical_10_runs": identical})
+             {"compute_type": "fp16", "plan": [sg.PLAN_VARIANTS[q[0]], q[1]], "bit_identical_10_runs": identical}, pw)
     sg.set_exponent_scaling(h, True)
     sg.destroy(h)
 

@@ -234,7 +234,7 @@ def _grouped_shapes(count, seed):
     return [(int(a), int(b), int(c)) for a, b, c in zip(m, n, k)]
 
 
-CLASS_PLAN = {0: (0, 4), 1: (1, 2), 2: (1, 3)}  # tile class -> (unsplit, split) single-call variant
+CLASS_PLAN = {0: (0, 4), 1: (1, 2), 2: (1, 3), 3: (1, 3)}  # tile class -> (unsplit, split) single-call variant
 
 
 def test_grouped_512_full_size(env):

@@ -770,8 +770,8 @@ def bench_energy(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the library runs every "
+              "problem on one GPU and does not shard across GPUs.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "grouped":
         bench_grouped(args)

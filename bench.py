@@ -311,7 +311,7 @@ def bench_grouped(args):
                                f"k logU[256,16384], A U[-1,1], B N(0,1/n), op_n/op_n fp16",
                    "problems": len(shapes), "total_flops": flops, "total_bytes": byts,
                    "work_units": units, "tile_classes": hist, "per_class": per_class,
-                   "classes_side_by_side": os.environ.get("MTK_SHGEMM_GROUP_CONCURRENT", "1") != "0",
+                   "classes_side_by_side": os.environ.get("MTK_SHGEMM_GROUP_CONCURRENT", "0") == "1",
                    "split_problems": int(sum(1 for s in slices if s > 1)), "parallelism": "single GPU"},
         "comparator": None if p_region is None else {
             "per_problem_launch_ms": round(p_region / max(2, args.steps // 2), 4),

@@ -7,7 +7,9 @@
 // Two owners of that sequence:
 //   * a group (create_group / run_group): fixed problems, one device
 //     allocation, the sequence captured once in a HIP graph whose tile
-//     classes run as parallel branches, each on a share of the GPU;
+//     classes run one after another, each on the whole GPU (or, with
+//     MTK_SHGEMM_GROUP_CONCURRENT=1, as parallel branches, each on a share of
+//     it: measured slower on the 512-problem batch, 9.2-9.8 against 9.0 ms);
 //   * a batch (create_batch / run_batch): preallocated slots, problems that
 //     change from call to call (a serving step's admitted requests), planned
 //     and launched eagerly on the caller's stream with no allocation.
@@ -425,7 +427,7 @@ struct group_t {
   size_t units = 0;
   char* dmem = nullptr;
   Launches launches;
-  bool concurrent = false;  // the graph runs the tile classes as parallel branches
+  bool concurrent = false;  // the graph runs the tile classes as parallel branches (else one after another)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -471,10 +473,10 @@ group_t* create_group(const shgemmHandle_t& handle, const operation_t op_a, cons
   }
 
   // ---- capture the launch sequence in one graph ----------------------------
-  // The tile classes run as parallel branches (MTK_SHGEMM_GROUP_CONCURRENT=0:
-  // one after another).
+  // The tile classes run one after another, each kernel filling the GPU
+  // (MTK_SHGEMM_GROUP_CONCURRENT=1: as parallel branches on shares of it).
   const char* cc = std::getenv("MTK_SHGEMM_GROUP_CONCURRENT");
-  g->concurrent = !(cc != nullptr && cc[0] == '0') && gemm.size() > 1;
+  g->concurrent = cc != nullptr && cc[0] == '1' && gemm.size() > 1;
   hipStream_t cs = nullptr;
   hipStream_t side[kNumGroupClasses] = {};
   hipEvent_t ev[kNumGroupClasses] = {};

@@ -135,13 +135,17 @@ def test_group_matches_single_calls_bitwise(h, variant, ops):
         sg.set_exponent_scaling(h, True)
 
 
-def test_graph_replays_match_eager(h):
+@pytest.mark.parametrize("concurrent", ["0", "1"])
+def test_graph_replays_match_eager(h, monkeypatch, concurrent):
     """Three graph replays with eager torch work between them, and the eager
-    launch sequence, all give the same bits."""
+    launch sequence, all give the same bits; the graph chains the tile
+    classes (the default) or runs them as parallel branches
+    (MTK_SHGEMM_GROUP_CONCURRENT=1, read when the group is created)."""
     import torch
 
     import shgemm_amd as sg
 
+    monkeypatch.setenv("MTK_SHGEMM_GROUP_CONCURRENT", concurrent)
     rng = np.random.default_rng(77)
     probs = [Prob(rng, 0, 0, 0, m, n, k, al, be) for m, n, k, al, be, _, _ in SPECS]
     with sg.Group(h, 0, 0, [p.tup() for p in probs], 0, 0, [s[5] for s in SPECS], [s[6] for s in SPECS]) as g:

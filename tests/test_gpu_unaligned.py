@@ -61,14 +61,19 @@ def _call(h, op_a, op_b, m, n, k, A, B, compute_type, lda, ldb, off):
 PLANS = [(0, 1), (1, 1), (2, 4), (3, 4), (4, 4), (5, 1), (9, 1), (10, 2), (11, 4), (8, 1), (-1, 1)]
 
 
+@pytest.mark.parametrize("shape", [(512, 512, 2112), (389, 333, 1999)])
 @pytest.mark.parametrize("op_a", [0, 1])
 @pytest.mark.parametrize("op_b", [0, 1])
 @pytest.mark.parametrize("compute_type", [FP16, TF32, FP16_CLASSIC])
-def test_unaligned_operands_same_bits(h, op_a, op_b, compute_type):
+def test_unaligned_operands_same_bits(h, op_a, op_b, compute_type, shape):
+    """(512, 512, 2112): whole tiles of every class, k a multiple of 64 so plan
+    8 applies; (389, 333, 1999): edge tiles in m and n and a k-tail, so the
+    unchecked interior loads, the bounds-checked edge loads and the tail run
+    on the same odd-ld operands."""
     import shgemm_amd as sg
 
-    m, n, k = 512, 512, 2112  # whole tiles of every class; k a multiple of 64 so plan 8 applies
-    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + compute_type)
+    m, n, k = shape
+    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + compute_type + 100 * (k % 2))
     ra, ca = (m, k) if op_a == 0 else (k, m)
     rb, cb = (k, n) if op_b == 0 else (n, k)
     A = rng.uniform(-1, 1, (ra, ca)).astype(np.float32)
@@ -83,7 +88,8 @@ def test_unaligned_operands_same_bits(h, op_a, op_b, compute_type):
             continue
         sg.force_plan(h, plan, slices)
         try:
-            ref = _call(h, op_a, op_b, m, n, k, A, B, compute_type, ra + 8, rb + 8, (0, 0))
+            # aligned reference: leading dimensions a multiple of 8 elements, aligned bases
+            ref = _call(h, op_a, op_b, m, n, k, A, B, compute_type, (ra + 15) // 8 * 8, (rb + 15) // 8 * 8, (0, 0))
             # odd leading dimensions and bases one element past 16-byte alignment
             got = _call(h, op_a, op_b, m, n, k, A, B, compute_type, ra + 1, rb + 1, (1, 1))
             got2 = _call(h, op_a, op_b, m, n, k, A, B, compute_type, ra + 3, rb + 5, (3, 3))

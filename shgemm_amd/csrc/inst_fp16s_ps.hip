@@ -1,14 +1,15 @@
-// Pre-split prescaled fp16 kernels (A split once per call, A and B packed in
-// tile order): C op_n and op_t, on the 256 x 256 tile of eight wavefronts.
+// Pre-split prescaled fp16 kernels (A split once per call and packed in tile
+// order; B packed the same way, or for op_b = op_n read as it lies): C op_n
+// and op_t, on the 256 x 256 tile of eight wavefronts.
 #include "register_shgemm.hpp"
 namespace shgemm_amd {
 using CfgPS = ModeCfg<MODE_FP16S>::PS;
-template <int OPC>
+template <int OPC, bool DIRECT_B>
 static void put_ps(KernelTable& t) {
-  static_assert(CfgPS::lds_bytes<MODE_FP16S>() <= kMaxLdsBytes, "tile exceeds the LDS of a CU");
-  KernelEntry& e = t.shgemm_ps[OPC];
-  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CfgPS>);
-  e.lds = CfgPS::lds_bytes<MODE_FP16S>();
+  static_assert(PsRingLayout<CfgPS>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
+  KernelEntry& e = t.shgemm_ps[DIRECT_B][OPC];
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CfgPS, DIRECT_B>);
+  e.lds = PsRingLayout<CfgPS>::lds_bytes();
   e.bm = CfgPS::BM;
   e.bn = CfgPS::BN;
   e.bk = CfgPS::BK;
@@ -18,7 +19,9 @@ static void put_ps(KernelTable& t) {
   e.wgs_per_cu = 1;
 }
 void register_shgemm_ps(KernelTable& t) {
-  put_ps<OP_N>(t);
-  put_ps<OP_T>(t);
+  put_ps<OP_N, false>(t);
+  put_ps<OP_T, false>(t);
+  put_ps<OP_N, true>(t);
+  put_ps<OP_T, true>(t);
 }
 }  // namespace shgemm_amd

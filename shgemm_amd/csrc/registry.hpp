@@ -47,8 +47,9 @@ struct KernelTable {
   KernelEntry sgemm[2][2][2];
   // deep-prefetch prescaled fp16 kernel (op_b = n, full k-tiles): [op_a][op_c]
   KernelEntry shgemm_bd[2][2];
-  // pre-split prescaled fp16 kernel (A split once per call, op_b = n): [op_c]
-  KernelEntry shgemm_ps[2];
+  // pre-split prescaled fp16 kernel (A split once per call): [B read as it
+  // lies (op_b = n) rather than packed][op_c]
+  KernelEntry shgemm_ps[2][2];
   // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
   KernelEntry shgemm_p0w[2][2][2];
   // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]

@@ -78,6 +78,7 @@ struct Plan {
   const shgemm_amd::KernelEntry* entry = nullptr;
   const char* note = "";
   bool presplit = false;  // A split once per call (plan variant 8)
+  bool direct_b = false;  // pre-split plan reading op_n B as it lies (no B pack pass)
 };
 
 // Largest useful split: every slice at least 4 k-tiles, all partials inside
@@ -128,8 +129,12 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
 
 // The pre-split kernel: prescaled fp16 and an even number of whole k-tiles
 // (A and B are packed by the plan's passes, whatever their layouts).
-const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int oc, int64_t k) {
-  const auto& e = st->table.shgemm_ps[oc];
+const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t k,
+                                        int64_t ldb, bool& direct_b) {
+  // op_n B is read in place when the tile's column offsets fit 32 bits, else packed
+  const auto& d = st->table.shgemm_ps[1][oc];
+  direct_b = ob == shgemm_amd::OP_N && (int64_t)d.bn * ldb * 2 < (int64_t(1) << 32);
+  const auto& e = st->table.shgemm_ps[direct_b ? 1 : 0][oc];
   if (e.fn != nullptr && mode == shgemm_amd::MODE_FP16S && k >= 2 * e.bk && k % (2 * e.bk) == 0) return &e;
   return nullptr;
 }
@@ -159,11 +164,13 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
       !ring_applies(tab[shgemm_amd::V_P2R], m, n, k, vec_a, vec_b, lda, ldb))
     requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
   if (requested == plan_variant_ps) {
-    pl.entry = ps_entry(st, mode, oc, k);
+    bool direct_b = false;
+    pl.entry = ps_entry(st, mode, ob, oc, k, ldb, direct_b);
     if (pl.entry != nullptr) {
       pl.variant = shgemm_amd::V_P2;
       pl.presplit = true;
-      pl.note = " pre-split";
+      pl.direct_b = direct_b;
+      pl.note = direct_b ? " pre-split A, B in place" : " pre-split";
     } else {
       pl.variant = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
     }
@@ -398,8 +405,9 @@ void create(shgemmHandle_t& handle) {
     for (auto& a : st->table.shgemm_bd)
       for (auto& e : a)
         if (e.fn) set_lds_limit(e);
-    for (auto& e : st->table.shgemm_ps)
-      if (e.fn) set_lds_limit(e);
+    for (auto& a : st->table.shgemm_ps)
+      for (auto& e : a)
+        if (e.fn) set_lds_limit(e);
     check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
     shgemm_amd::init_static_scratch();
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
@@ -588,11 +596,12 @@ detail::kernel_level shgemm_call(const shgemmHandle_t handle, const operation_t 
   }
   if (plan.presplit) {
     // A's packed hi and lo tiles (4 bytes per element, m rounded up to 256)
-    // and B's packed tiles (2 bytes per element, n rounded up to 256), in a
-    // handle buffer grown on demand; where the device cannot hold them, or
-    // the buffer would have to grow during graph capture, the call runs the
-    // register-split 256 x 256 tile (bit-identical).
-    const size_t bytes = shgemm_amd::presplit_bytes(p.m, p.k) + shgemm_amd::pack_b_bytes(p.n, p.k);
+    // and, for op_b = op_t, B's packed tiles (2 bytes per element, n rounded
+    // up to 256), in a handle buffer grown on demand; where the device cannot
+    // hold them, or the buffer would have to grow during graph capture, the
+    // call runs the register-split 256 x 256 tile (bit-identical).
+    const size_t bytes =
+        shgemm_amd::presplit_bytes(p.m, p.k) + (plan.direct_b ? 0 : shgemm_amd::pack_b_bytes(p.n, p.k));
     if (!ensure_presplit(st, bytes, capturing())) {
       plan.presplit = false;
       plan.entry = nullptr;
@@ -636,11 +645,13 @@ detail::kernel_level shgemm_call(const shgemmHandle_t handle, const operation_t 
     uint4* bpack = reinterpret_cast<uint4*>(reinterpret_cast<char*>(st->ps_buf) + a_bytes);
     check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp, apack, stream),
               "pre-split pass");
-    check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k, bpack,
-                                        stream),
-              "B pack pass");
     p.a = reinterpret_cast<const float*>(apack);  // the kernel reads the packed tiles
-    p.b = bpack;
+    if (!plan.direct_b) {
+      check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k,
+                                          bpack, stream),
+                "B pack pass");
+      p.b = bpack;
+    }
   }
   p.tiles_m = (int)((m + e.bm - 1) / e.bm);
   p.tiles_n = (int)((n + e.bn - 1) / e.bn);

@@ -382,6 +382,19 @@ __device__ __forceinline__ constexpr int vmcnt_imm(int n) {
   return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
 }
 
+// One wavefront instruction of 16-byte global -> LDS copies: lane l's 16
+// bytes at src land at LDS address dst + 16 * l (dst wavefront-uniform).
+__device__ __forceinline__ void lds_dma16(const char* src, uint4* dst) {
+  __builtin_amdgcn_global_load_lds(src, reinterpret_cast<uint32_t*>(dst), 16, 0, 0);
+}
+
+template <int N>
+__device__ __forceinline__ void lds_ring_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4));  // vmcnt(N) lgkmcnt(0)
+  asm volatile("s_barrier" ::: "memory");
+}
+
 // Running row exponents of one output tile (LDS, after the two stages):
 // by k-tile parity the maxima |a| (bits, LDS atomic max) of the chunks that
 // exceed their row's exponent and the index of the last k-tile with such a
@@ -495,10 +508,9 @@ struct ModeCfg<MODE_FP16S> {
   // The pre-split plan's tile (plan 8): 256 x 256 as eight 64 x 128
   // wavefronts, two per SIMD, so one wavefront's barrier and LDS waits are
   // covered by the other's MFMAs (the four 128 x 128 wavefronts of P2 run one
-  // per SIMD).  Two k-tiles of loads in registers, the skewed schedule; the
-  // same fragments per k-step and the same bits as P2.  16384^3 kernel
-  // 14.50 against 14.82 ms, 8192^3 1.76 against 1.82 ms, 4096^3 0.237
-  // against 0.251 ms (one box, interleaved).
+  // per SIMD); the same fragments per k-step and the same bits as P2.  (PF
+  // and IL describe the register-split P2 form; the pre-split kernel stages
+  // its loads in an LDS ring.)
   using PS = TileCfg<256, 256, 32, 64, 128, 2, 2, true>;
   static constexpr bool kHasP2 = true;
 };
@@ -1277,8 +1289,7 @@ __device__ __forceinline__ void shgemm_tile_ring(const GemmParams& p, int m0, in
 
   issue(0);
   issue(1);
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));  // tile 0 landed (tile 1 may be in flight)
-  __syncthreads();
+  lds_ring_barrier<PER>();  // tile 0 landed (tile 1 may be in flight)
   for (int kt = 0; kt < ktiles; ++kt) {
     // the slot of tile kt+2 held tile kt-1, whose reads ended at the last barrier
     if (kt + 2 < ktiles) issue(kt + 2);
@@ -1286,9 +1297,8 @@ __device__ __forceinline__ void shgemm_tile_ring(const GemmParams& p, int m0, in
     compute(kt);
     __builtin_amdgcn_sched_barrier(0);
     // tile kt+1 landed once at most tile kt+2's loads are outstanding
-    if (kt + 2 < ktiles) __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));
-    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-    __syncthreads();
+    if (kt + 2 < ktiles) lds_ring_barrier<PER>();
+    else lds_ring_barrier<0>();
   }
   f32x16 out[1][FM][FN];
 #pragma unroll
@@ -1310,68 +1320,90 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ring(const Ge
   shgemm_tile_ring<OPC, CFG>(p, tm * CFG::BM, tn * CFG::BN, lds);
 }
 
-// A tile of the pre-split operand: SLOTS 16-byte slots already in LDS-image
-// order, copied by NT threads (slot tid + NT * p).
-template <int SLOTS, int NT>
-struct PackedTile {
-  static constexpr int PER = SLOTS / NT;
-  static_assert(SLOTS % NT == 0, "whole slots per thread");
-  u32x4 v[PER];
-  __device__ __forceinline__ void load(const uint4* __restrict__ src, int tid) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) v[p] = *reinterpret_cast<const u32x4*>(&src[tid + NT * p]);
-  }
-  __device__ __forceinline__ void store(uint4* __restrict__ dst, int tid) const {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) *reinterpret_cast<u32x4*>(&dst[tid + NT * p]) = v[p];
-  }
-};
-
 // ---------------------------------------------------------------------------
 // Pre-split kernel of the prescaled fp16 mode (256 x 256 x 32 tile; the
-// library uses eight wavefronts of 64 x 128, ModeCfg::PS).  A arrives already split and packed
-// (launch_presplit_a: per 256-row block and k-tile, 32 KiB holding the hi
-// then lo LDS images; p.a points at it) and B packed the same way
-// (launch_pack_b: 16 KiB per 256-column block and k-tile; p.b points at
-// it), so a k-tile is one contiguous A block and one contiguous B block:
-// 16-byte loads, two k-tiles ahead in two register sets, stored to LDS
-// unchanged -- no VALU split.  Per k-tile (stage cur holds
-// tile kt, one register set holds tile kt+1, the other receives tile kt+2):
-//   loads of tile kt+2 || reads of step 1 || MFMAs of step 0 || stores of
-//   tile kt+1 into stage cur^1;  barrier;  reads of step 0 of tile kt+1 ||
-//   MFMAs of step 1.
+// library uses eight wavefronts of 64 x 128, ModeCfg::PS).  A arrives already
+// split and packed (launch_presplit_a: per 256-row block and k-tile, 32 KiB
+// holding the hi then lo LDS images; p.a points at it), so a k-tile of A is
+// one contiguous block and needs no VALU split.  Every operand byte goes from
+// global memory straight to LDS (global_load_lds_dwordx4) into a 3-stage
+// ring, issued two k-tiles ahead of the MFMAs that use it.  No register holds
+// a load in flight, so the accumulators, both fragment sets and the addresses
+// fit the register budget of two wavefronts per SIMD; staging the loads in
+// registers instead spills, and each spill reload waits for every load in
+// flight (16384^3: kernel 13.1-13.5 against 13.9 ms).  B is the packed image
+// (launch_pack_b) or, with DIRECT_B, the caller's op_n B read in place: the
+// lane at slot s of the image fetches column s / KC, k-chunk (s % KC) ^
+// lds_key_h<KC>(s / KC), so four lanes read one column's 64 contiguous bytes
+// per k-tile; columns past n read column n - 1 (they feed only C columns past
+// n, which are never stored) and the host guarantees BN * ldb * 2 < 2^32.
+// Per k-tile kt in stage kt % 3:
+//   issue tile kt+2 into stage (kt+2) % 3 (it held tile kt-1, whose reads
+//   completed before the last barrier) || reads of step 1 || MFMAs of
+//   step 0;  wait until tile kt+1 has landed;  barrier;  reads of step 0 of
+//   tile kt+1 || MFMAs of step 1.
 // Each C element accumulates in the order of the register-split P2 tile, so
-// the results are bit-identical to it.  Rows past m and columns past n are
-// zero in the packed operands (never stored); the host guarantees k % 64 ==
-// 0 (an even number of k-tiles).
+// the results are bit-identical to it.  The host guarantees k % 64 == 0.
 // ---------------------------------------------------------------------------
-template <int OPC, class CFG>
-__device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __restrict__ lds) {
-  constexpr int BM = CFG::BM, BN = CFG::BN, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
-  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM;
+// Workgroup barrier for an LDS ring filled by global_load_lds: wait until at
+// most N of this thread's vector-memory operations are outstanding (the
+// older ones, whose LDS writes have then landed) and its LDS reads have
+// completed, then s_barrier.  No memory fence: a workgroup release fence
+// would also wait for the N newest loads, the ones meant to stay in flight.
+// The asm statement's memory clobber keeps the compiler from moving memory
+// accesses across the barrier.
+template <class CFG>
+struct PsRingLayout {
+  static constexpr int A_CH = CFG::BM * CFG::KC, B_CH = CFG::BN * CFG::KC;
+  static constexpr int STAGE = 2 * A_CH + B_CH;  // 16-byte slots: A hi, A lo, B
+  static constexpr int NSTAGE = 3;
+  static constexpr size_t lds_bytes() { return (size_t)NSTAGE * STAGE * 16; }
+};
+
+template <int OPC, class CFG, bool DIRECT_B>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const GemmParams p) {
+  constexpr int BM = CFG::BM, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM, WAVES = NT / 64;
   constexpr bool LANE_M = (OPC == OP_N);
   static_assert(KC == 4, "two k-steps per k-tile");
-  constexpr int A_CH = BM * KC, B_CH = BN * KC;
-  constexpr int STAGE = 2 * A_CH + B_CH;
+  using L = PsRingLayout<CFG>;
+  constexpr int A_CH = L::A_CH, B_CH = L::B_CH, STAGE = L::STAGE;
+  static_assert(A_CH % NT == 0 && B_CH % NT == 0, "whole wavefront instructions per thread");
+  constexpr int QA = A_CH / NT, QB = B_CH / NT;  // instructions per thread: each A half, B
+  constexpr int PER = 2 * QA + QB;               // vector-memory instructions per thread per k-tile
   using Frag = FragF16<FM, FN>;
   constexpr int MF = FM * FN * 2;  // MFMAs per k-step
   constexpr int NR = 2 * FM + FN;  // fragment reads per k-step
-  using TA = PackedTile<A_CH, NT>;
-  using TB = PackedTile<B_CH, NT>;
-  constexpr int NG = 2 * TA::PER + TB::PER;  // global loads per k-tile
-  constexpr int NW = NG;                     // LDS stores per k-tile
+  static_assert(PER <= NR && NR <= MF, "every load and fragment read has an MFMA beside it");
+  extern __shared__ uint4 lds[];
 
   int tm, tn;
   tile_coords(p, blockIdx.x, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = tm * CFG::BM, n0 = tn * CFG::BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
   const int ktiles = p.k / BK;
-  // this block row's packed k-tiles: 2 * A_CH slots each (hi, then lo)
-  const uint4* __restrict__ apack = reinterpret_cast<const uint4*>(p.a) + (int64_t)tm * ktiles * (2 * A_CH);
-  // this block column's packed B k-tiles: B_CH slots each
-  const uint4* __restrict__ bpack = static_cast<const uint4*>(p.b) + (int64_t)tn * ktiles * B_CH;
+  // this block row's packed k-tiles (2 * A_CH slots each: hi, then lo)
+  const char* const apack =
+      uniform_ptr(reinterpret_cast<const char*>(p.a) + (int64_t)tm * ktiles * (2 * A_CH) * 16);
+  // B: this block column's packed k-tiles (B_CH slots each), or column n0 of the caller's B
+  const char* const bbase =
+      DIRECT_B ? uniform_ptr(reinterpret_cast<const char*>(static_cast<const uint16_t*>(p.b) + (int64_t)n0 * p.ldb))
+               : uniform_ptr(reinterpret_cast<const char*>(p.b) + (int64_t)tn * ktiles * B_CH * 16);
+  constexpr int64_t B_STEP = DIRECT_B ? BK * 2 : B_CH * 16;  // bytes from one k-tile of B to the next
+  uint32_t boff[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int s = (q * WAVES + wave) * 64 + lane;
+    if constexpr (DIRECT_B) {
+      const int j = s / KC, c = (s % KC) ^ lds_key_h<KC>(j);
+      boff[q] = (uint32_t)(((int64_t)(min(n0 + j, p.n - 1) - n0) * p.ldb + 8 * c) * 2);
+    } else {
+      boff[q] = (uint32_t)s * 16u;
+    }
+  }
 
   f32x16 acc[1][FM][FN];
 #pragma unroll
@@ -1381,52 +1413,48 @@ __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __res
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[0][i][j][e] = 0.0f;
 
-  TA th[2], tl[2];
-  TB tb[2];
-  auto gload = [&](auto set_c, int kt) {
-    constexpr int set = decltype(set_c)::value;
-    const uint4* at = apack + (int64_t)kt * (2 * A_CH);
-    th[set].load(at, tid);
-    tl[set].load(at + A_CH, tid);
-    tb[set].load(bpack + (int64_t)kt * B_CH, tid);
-  };
-  auto sstore = [&](auto set_c, int stage) {
-    constexpr int set = decltype(set_c)::value;
-    uint4* st = lds + stage * STAGE;
-    th[set].store(st, tid);
-    tl[set].store(st + A_CH, tid);
-    tb[set].store(st + 2 * A_CH, tid);
+  // tile kt into stage st: lane l of instruction (q, wave) fills slot (q * WAVES + wave) * 64 + l
+  auto issue = [&](int kt, int st) {
+    uint4* const dst = lds + st * STAGE;
+    const char* const at = apack + (int64_t)kt * (2 * A_CH * 16);
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      const int blk = q * WAVES + wave;
+      const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
+      lds_dma16(at + off, dst + blk * 64);
+      lds_dma16(at + A_CH * 16 + off, dst + A_CH + blk * 64);
+    }
+    const char* const bt = bbase + (int64_t)kt * B_STEP;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int blk = q * WAVES + wave;
+      lds_dma16(bt + boff[q], dst + 2 * A_CH + blk * 64);
+    }
   };
   Frag f0, f1;
-  auto fload = [&](Frag& f, int stage, int step) {
-    const uint4* st = lds + stage * STAGE;
-    frag_load<FM, FN, KC, BM, WM, WN>(f, st, st + 2 * A_CH, step, wm, wn, lrow, lh);
+  auto fload = [&](Frag& f, int st, int step) {
+    const uint4* const src = lds + st * STAGE;
+    frag_load<FM, FN, KC, BM, WM, WN>(f, src, src + 2 * A_CH, step, wm, wn, lrow, lh);
   };
-  // SETL: the register set that receives tile kt+2 (it held tile kt); the
-  // other set holds tile kt+1.  MORE: tile kt+2 exists; NEXT: tile kt+1 does.
-  auto ktile = [&](int kt, auto setl_c, auto more_c, auto next_c) {
-    constexpr int SETL = decltype(setl_c)::value;
+  // k-tile kt in stage cur; MORE: tile kt+2 exists; NEXT: tile kt+1 does
+  auto ktile = [&](int kt, int cur, auto more_c, auto next_c) {
     constexpr bool MORE = decltype(more_c)::value, NEXT = decltype(next_c)::value;
-    using SN = std::integral_constant<int, 1 - SETL>;
-    const int cur = kt & 1;
-    if constexpr (MORE) gload(setl_c, kt + 2);
+    const int nxt = cur == 2 ? 0 : cur + 1, far = cur == 0 ? 2 : cur - 1;
+    if constexpr (MORE) issue(kt + 2, far);
     fload(f1, cur, 1);
     frag_mma<LANE_M>(acc, f0);
-    if constexpr (NEXT) sstore(SN{}, cur ^ 1);
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      if (MORE && q < PER) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
     }
-#pragma unroll
-    for (int q = NR; q < MF; ++q) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-    }
+    __builtin_amdgcn_sched_group_barrier(0x008, MF - NR, 0);
     __builtin_amdgcn_sched_barrier(0);
-    __syncthreads();
-    if constexpr (NEXT) fload(f0, cur ^ 1, 0);
+    // tile kt+1 has landed once at most tile kt+2's loads are outstanding
+    if constexpr (MORE) lds_ring_barrier<PER>();
+    else lds_ring_barrier<0>();
+    if constexpr (NEXT) fload(f0, nxt, 0);
     frag_mma<LANE_M>(acc, f1);
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -1436,30 +1464,20 @@ __device__ __forceinline__ void shgemm_tile_ps(const GemmParams& p, uint4* __res
     __builtin_amdgcn_sched_group_barrier(0x008, MF - NR, 0);
     __builtin_amdgcn_sched_barrier(0);
   };
-  static_assert(NW <= MF - NR, "every LDS store has an MFMA to follow");
-  using S0 = std::integral_constant<int, 0>;
-  using S1 = std::integral_constant<int, 1>;
   using T = std::true_type;
   using F = std::false_type;
-  gload(S0{}, 0);
-  gload(S1{}, 1);  // ktiles >= 2
-  sstore(S0{}, 0);
-  __syncthreads();
+  issue(0, 0);
+  issue(1, 1);  // ktiles >= 2
+  lds_ring_barrier<PER>();  // tile 0 landed
   fload(f0, 0, 0);
-  int kt = 0;
-  for (; kt + 3 < ktiles; kt += 2) {
-    ktile(kt, S0{}, T{}, T{});
-    ktile(kt + 1, S1{}, T{}, T{});
+  int kt = 0, cur = 0;
+  for (; kt + 2 < ktiles; ++kt) {
+    ktile(kt, cur, T{}, T{});
+    cur = cur == 2 ? 0 : cur + 1;
   }
-  ktile(kt, S0{}, F{}, T{});  // ktiles is even
-  ktile(kt + 1, S1{}, F{}, F{});
+  ktile(kt, cur, F{}, T{});
+  ktile(kt + 1, cur == 2 ? 0 : cur + 1, F{}, F{});
   store_ctile<MODE_FP16S, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
-}
-
-template <int OPC, class CFG>
-__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const GemmParams p) {
-  extern __shared__ uint4 lds[];
-  shgemm_tile_ps<OPC, CFG>(p, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1534,9 +1552,8 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_bd(const Gemm
   // its B load must have landed; at most one newer k-tile of loads may still
   // be in flight (`newer`).
   auto publish = [&](bool newer) {
-    if (newer) __builtin_amdgcn_s_waitcnt(vmcnt_imm(VM_PER_TILE));
-    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-    __syncthreads();
+    if (newer) lds_ring_barrier<VM_PER_TILE>();
+    else lds_ring_barrier<0>();
   };
 
   if (ktiles > 0) {

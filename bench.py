@@ -938,7 +938,7 @@ def main():
                      "note": "achieved = 2mnk / mean duration of the main GEMM kernel alone (HIP events the library "
                              "records around it on the launch stream); left out of that duration and counted in "
                              "ms_per_step: the row-exponent pass over A and, on the pre-split plan, the pack "
-                             "pass of A (split into FP16 hi/lo) and of B; peak = FP16 dense 2516.6 / 2, since "
+                             "pass of A (split into FP16 hi/lo) and, for op_t B only, of B; peak = FP16 dense 2516.6 / 2, since "
                              "every algorithmic FMA costs two FP16 MFMA products (hi and lo); "
                              "frac_of_fp16_dense_peak = achieved / 2516.6; frac_at_measured_clock = achieved / "
                              "(peak x median GFX clock of the call's power window / 2400 MHz, the clock of the "

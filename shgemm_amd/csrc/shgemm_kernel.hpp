@@ -121,7 +121,7 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
         int r, c;
         M::item(tid + NT * p, r, c);
         const int rr = min(r0 + r, rlim - 1) - r0;
-        const char* tile = uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)r0 * ld + k0));
+        const char* tile = as_global(uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)r0 * ld + k0)));
         const uint32_t boff = 4u * ((uint32_t)rr * (uint32_t)ld + 8u * (uint32_t)c);
         const f32x4 x = *reinterpret_cast<const f32x4*>(tile + boff);
         const f32x4 y = *reinterpret_cast<const f32x4*>(tile + boff + 16);
@@ -138,7 +138,7 @@ struct TileF32 : ItemMap<KCONT, R, BK, NT> {
         const uint32_t rbytes = 4u * (uint32_t)min(r0 + r, rlim - 1);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const char* col = uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)(k0 + 8 * c + e) * ld));
+          const char* col = as_global(uniform_ptr(reinterpret_cast<const char*>(base + (int64_t)(k0 + 8 * c + e) * ld)));
           v[p][e] = *reinterpret_cast<const float*>(col + rbytes);
         }
       }
@@ -1522,8 +1522,8 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_bd(const Gemm
   tile_coords(p, blockIdx.x, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int ktiles = p.k / BK;
-  const float* __restrict__ A = p.a;
-  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(p.b);
+  const float* __restrict__ A = as_global(p.a);  // global_* (not flat_*) loads: a flat load waits for the LDS DMA in flight
+  const uint16_t* __restrict__ B = as_global(static_cast<const uint16_t*>(p.b));
   ta0.init_scale(p.row_exp, m0, p.m, tid);
   ta1.init_scale(p.row_exp, m0, p.m, tid);
 

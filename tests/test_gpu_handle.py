@@ -14,6 +14,10 @@ Bars:
     offsets would not fit 32 bits (op_b = N with ldb = k >= 2^23, op_t A
     with lda = k >= 2^23), and stays plan 7 below that (query only: nothing
     of that size is launched);
+  * plan 8 (pre-split) reads op_n B where it lies while the 256-column
+    tile's byte offsets fit 32 bits (256 * ldb * 2 < 2^32) and packs it
+    otherwise; on both sides of that limit it gives plan 5's bits (B of
+    three columns at ldb = 2^23 - 8 and 2^23: a 16 MiB buffer);
   * set_debug_mode prints the reference's debug line for each call.
 """
 from __future__ import annotations
@@ -220,4 +224,39 @@ def test_debug_mode_prints_call_line(env, capfd):
         assert len(lines) == 1, out
         assert lines[0].startswith("[shape=(300,200,100),op_a=T,op_b=N,compute_type=FP16]"), lines[0]
     finally:
+        sg.destroy(h)
+
+
+@pytest.mark.parametrize("ldb", [(1 << 23) - 8, 1 << 23])
+def test_presplit_reads_b_in_place_below_the_offset_limit(env, capfd, ldb):
+    torch, sg = env
+    h = sg.create()
+    try:
+        m, n, k = 300, 3, 128
+        rng = np.random.default_rng(ldb % 97)
+        A = torch.from_numpy(rng.uniform(-1, 1, m * k).astype(np.float32)).cuda()
+        cols = torch.from_numpy(rng.standard_normal((n, k)).astype(np.float16)).cuda()
+        B = torch.zeros((n - 1) * ldb + k, dtype=torch.float16, device="cuda")
+        for j in range(n):
+            B[j * ldb:j * ldb + k] = cols[j]
+        out = {}
+        for plan in (PLAN_PS, PLAN_P2):
+            C = torch.full((m * n,), float("nan"), device="cuda")
+            sg.force_plan(h, plan, 1)
+            sg.set_debug_mode(h, 1)
+            sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, ldb, 0.0, C, m, FP16)
+            torch.cuda.synchronize()
+            sg.set_debug_mode(h, 0)
+            out[plan] = C.cpu().numpy()
+        lines = [l for l in capfd.readouterr().out.splitlines() if l.startswith("[shape=")]
+        assert len(lines) == 2, lines
+        in_place = 256 * ldb * 2 < (1 << 32)
+        assert ("pre-split A, B in place" in lines[0]) == in_place, lines[0]
+        assert "pre-split" in lines[0], lines[0]
+        assert np.array_equal(out[PLAN_PS].view(np.uint32), out[PLAN_P2].view(np.uint32))
+        ref = A.cpu().numpy().astype(np.float64).reshape(k, m).T @ cols.cpu().numpy().astype(np.float64).T
+        got = out[PLAN_PS].reshape(n, m).T
+        assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    finally:
+        sg.force_plan(h, -1, 1)
         sg.destroy(h)

@@ -416,13 +416,17 @@ hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int ns
 // 8c..8c+7 (the LDS image of the tile), rows past m zero.  A GEMM workgroup
 // then streams one contiguous 32 KiB block per k-tile.
 // Caller: k % 32 == 0.  One workgroup of 256 threads per tile.
+// BF16 (the tf32 compute type): the three BF16 parts a0, a1, a2 of each
+// element (split_bf16x3, as the register-split kernels make them), no row
+// exponents; a tile is 48 KiB, the a0, a1 then a2 images.
 // ---------------------------------------------------------------------------
 constexpr int kPackRows = 256, kPackK = 32, kPackSlots = kPackRows * kPackK / 8;
 
-template <bool KCONT>
+template <bool KCONT, bool BF16>
 __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
                                                             int vec, const int* __restrict__ row_exp,
                                                             uint4* __restrict__ out) {
+  constexpr int PLANES = BF16 ? 3 : 2;
   __shared__ float t[KCONT ? 1 : kPackK][kPackRows + 1];
   const int mb = (m + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
   const int bm = (int)(blockIdx.x % mb), kt = (int)(blockIdx.x / mb);
@@ -450,7 +454,7 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
     }
     __syncthreads();
   }
-  uint4* tile = out + ((int64_t)bm * kt_n + kt) * (2 * kPackSlots);
+  uint4* tile = out + ((int64_t)bm * kt_n + kt) * (PLANES * kPackSlots);
 #pragma unroll
   for (int q = 0; q < kPackSlots / 256; ++q) {
     const int s = tid + 256 * q;
@@ -465,14 +469,29 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
 #pragma unroll
       for (int e = 0; e < 8; ++e) v[e] = t[8 * c + e][r];
     }
-    const float sc = r0 + r < m ? __builtin_ldexpf(1.0f, -row_exp[r0 + r]) : 1.0f;
-    uint4 h, l;
-    split2_f16_scaled(v[0], v[1], sc, h.x, l.x);
-    split2_f16_scaled(v[2], v[3], sc, h.y, l.y);
-    split2_f16_scaled(v[4], v[5], sc, h.z, l.z);
-    split2_f16_scaled(v[6], v[7], sc, h.w, l.w);
-    tile[s] = h;
-    tile[kPackSlots + s] = l;
+    if constexpr (BF16) {
+      bf16x8 x0, x1, x2;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 a0, a1, a2;
+        split_bf16x3(v[e], a0, a1, a2);
+        x0[e] = a0;
+        x1[e] = a1;
+        x2[e] = a2;
+      }
+      tile[s] = __builtin_bit_cast(uint4, x0);
+      tile[kPackSlots + s] = __builtin_bit_cast(uint4, x1);
+      tile[2 * kPackSlots + s] = __builtin_bit_cast(uint4, x2);
+    } else {
+      const float sc = r0 + r < m ? __builtin_ldexpf(1.0f, -row_exp[r0 + r]) : 1.0f;
+      uint4 h, l;
+      split2_f16_scaled(v[0], v[1], sc, h.x, l.x);
+      split2_f16_scaled(v[2], v[3], sc, h.y, l.y);
+      split2_f16_scaled(v[4], v[5], sc, h.z, l.z);
+      split2_f16_scaled(v[6], v[7], sc, h.w, l.w);
+      tile[s] = h;
+      tile[kPackSlots + s] = l;
+    }
   }
 }
 
@@ -481,10 +500,13 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
 // lds_slot_h<4>(j, c) = column j's k-values 8c..8c+7, FP16 bits unchanged),
 // columns past n zero.  B is k x n with op_b applied: op_n stores element
 // (kk, j) at b[kk + j * ldb] (k contiguous), op_t at b[j + kk * ldb].  Loads
-// are 16 bytes at any ldb and base alignment.
-template <bool KCONT>
+// are 16 bytes at any ldb and base alignment.  BF16 (the tf32 compute type):
+// the two exact BF16 parts b0, b1 of each value (split_bf16x2), 32 KiB per
+// tile, the b0 then b1 images.
+template <bool KCONT, bool BF16>
 __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict__ b, int64_t ldb, int n, int k,
                                                      uint4* __restrict__ out) {
+  constexpr int PLANES = BF16 ? 2 : 1;
   __shared__ uint16_t t[KCONT ? 1 : kPackK][kPackRows + 8];
   const int nb = (n + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
   const int bn = (int)(blockIdx.x % nb), kt = (int)(blockIdx.x / nb);
@@ -506,7 +528,7 @@ __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict_
     }
     __syncthreads();
   }
-  uint4* tile = out + ((int64_t)bn * kt_n + kt) * kPackSlots;
+  uint4* tile = out + ((int64_t)bn * kt_n + kt) * (PLANES * kPackSlots);
 #pragma unroll
   for (int q = 0; q < kPackSlots / 256; ++q) {
     const int s = tid + 256 * q;
@@ -521,43 +543,55 @@ __global__ void __launch_bounds__(256) pack_b_kernel(const uint16_t* __restrict_
       for (int e = 0; e < 4; ++e) h[e] = (uint32_t)t[8 * c + 2 * e][j] | ((uint32_t)t[8 * c + 2 * e + 1][j] << 16);
       w = uint4{h[0], h[1], h[2], h[3]};
     }
-    tile[s] = w;
+    if constexpr (BF16) {
+      bf16x8 x0, x1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 b0, b1;
+        split_bf16x2(bits_to_f16((uint16_t)((w[e >> 1] >> (16 * (e & 1))) & 0xffffu)), b0, b1);
+        x0[e] = b0;
+        x1[e] = b1;
+      }
+      tile[s] = __builtin_bit_cast(uint4, x0);
+      tile[kPackSlots + s] = __builtin_bit_cast(uint4, x1);
+    } else {
+      tile[s] = w;
+    }
   }
 }
 
-size_t pack_b_bytes(int n, int k) {
-  return (size_t)((n + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * 2;
+size_t pack_b_bytes(int n, int k, int mode) {
+  return (size_t)((n + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * (mode == MODE_BF16 ? 4 : 2);
 }
 
-hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, uint4* out, hipStream_t s) {
+hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, int mode, uint4* out,
+                         hipStream_t s) {
   if (n <= 0 || k <= 0) return hipSuccess;
   if (k % kPackK != 0) return hipErrorInvalidValue;
   const int64_t blocks = (int64_t)((n + kPackRows - 1) / kPackRows) * (k / kPackK);
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
-  if (op_b == OP_N)
-    hipLaunchKernelGGL(pack_b_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, out);
-  else
-    hipLaunchKernelGGL(pack_b_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, out);
+  const bool bf16 = mode == MODE_BF16;
+  auto fn = op_b == OP_N ? (bf16 ? pack_b_kernel<true, true> : pack_b_kernel<true, false>)
+                         : (bf16 ? pack_b_kernel<false, true> : pack_b_kernel<false, false>);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, s, b, ldb, n, k, out);
   return hipGetLastError();
 }
 
-size_t presplit_bytes(int m, int k) {
-  return (size_t)((m + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * 4;
+size_t presplit_bytes(int m, int k, int mode) {
+  return (size_t)((m + kPackRows - 1) / kPackRows) * kPackRows * (size_t)k * (mode == MODE_BF16 ? 6 : 4);
 }
 
-hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, const int* row_exp, uint4* out,
-                             hipStream_t s) {
+hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, int mode, const int* row_exp,
+                             uint4* out, hipStream_t s) {
   if (m <= 0 || k <= 0) return hipSuccess;
   if (k % kPackK != 0) return hipErrorInvalidValue;
   const int64_t blocks = (int64_t)((m + kPackRows - 1) / kPackRows) * (k / kPackK);
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
   const int vec = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (lda % 4 == 0);
-  if (op_a == OP_T)
-    hipLaunchKernelGGL(presplit_pack_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp,
-                       out);
-  else
-    hipLaunchKernelGGL(presplit_pack_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp,
-                       out);
+  const bool bf16 = mode == MODE_BF16;
+  auto fn = op_a == OP_T ? (bf16 ? presplit_pack_kernel<true, true> : presplit_pack_kernel<true, false>)
+                         : (bf16 ? presplit_pack_kernel<false, true> : presplit_pack_kernel<false, false>);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp, out);
   return hipGetLastError();
 }
 

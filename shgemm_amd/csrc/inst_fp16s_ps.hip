@@ -18,7 +18,26 @@ static void put_ps(KernelTable& t) {
   e.needs_row_exp = true;
   e.wgs_per_cu = 1;
 }
+// tf32 (BF16 parts): the same 256 x 256 tile of eight 64 x 128 wavefronts,
+// one workgroup per CU.
+using CfgPSB = ModeCfg<MODE_BF16>::PS;
+template <int OPC>
+static void put_ps_bf16(KernelTable& t) {
+  static_assert(PsBf16Layout<CfgPSB>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
+  KernelEntry& e = t.shgemm_ps_bf16[OPC];
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps_bf16<OPC, CfgPSB>);
+  e.lds = PsBf16Layout<CfgPSB>::lds_bytes();
+  e.bm = CfgPSB::BM;
+  e.bn = CfgPSB::BN;
+  e.bk = CfgPSB::BK;
+  e.mode = MODE_BF16;
+  e.threads = CfgPSB::NT;
+  e.needs_row_exp = false;
+  e.wgs_per_cu = 1;
+}
 void register_shgemm_ps(KernelTable& t) {
+  put_ps_bf16<OP_N>(t);
+  put_ps_bf16<OP_T>(t);
   put_ps<OP_N, false>(t);
   put_ps<OP_T, false>(t);
   put_ps<OP_N, true>(t);

@@ -50,6 +50,8 @@ struct KernelTable {
   // pre-split prescaled fp16 kernel (A split once per call): [B read as it
   // lies (op_b = n) rather than packed][op_c]
   KernelEntry shgemm_ps[2][2];
+  // pre-split tf32 kernel (A's three BF16 parts and B's two packed once per call): [op_c]
+  KernelEntry shgemm_ps_bf16[2];
   // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
   KernelEntry shgemm_p0w[2][2][2];
   // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]
@@ -138,16 +140,20 @@ hipError_t launch_scale_c(int64_t elems, int inner, float beta, float* c, int64_
 hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, int k, uint32_t* row_max,
                                 int* row_exp, uint32_t* partials, size_t partial_bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
-// Prescaled FP16 hi / residual lo of A in the pre-split plan's tile order
-// (256-row blocks x 32-deep k-tiles, 32 KiB per tile); k % 32 == 0.  The
-// output needs presplit_bytes(m, k) bytes.
-size_t presplit_bytes(int m, int k);
-hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, const int* row_exp, uint4* out,
-                             hipStream_t s);
+// A split in the pre-split plan's tile order (256-row blocks x 32-deep
+// k-tiles): prescaled FP16 hi / residual lo (MODE_FP16S, 32 KiB per tile,
+// row_exp from the row pass) or three BF16 parts (MODE_BF16, 48 KiB per
+// tile, row_exp unused); k % 32 == 0.  The output needs
+// presplit_bytes(m, k, mode) bytes.
+size_t presplit_bytes(int m, int k, int mode);
+hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, int mode, const int* row_exp,
+                             uint4* out, hipStream_t s);
 // B (FP16, k x n after op_b) in the same tile order: 256-column blocks x
-// 32-deep k-tiles, 16 KiB per tile; k % 32 == 0; pack_b_bytes(n, k) bytes.
-size_t pack_b_bytes(int n, int k);
-hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, uint4* out, hipStream_t s);
+// 32-deep k-tiles, 16 KiB per tile (MODE_FP16S: the FP16 bits) or 32 KiB
+// (MODE_BF16: two BF16 parts); k % 32 == 0; pack_b_bytes(n, k, mode) bytes.
+size_t pack_b_bytes(int n, int k, int mode);
+hipError_t launch_pack_b(const uint16_t* b, int64_t ldb, int op_b, int n, int k, int mode, uint4* out,
+                         hipStream_t s);
 hipError_t launch_row_exp(const uint32_t* row_max, int64_t rows, int* row_exp, hipStream_t s);
 hipError_t launch_grouped_rowmax(const RowmaxSeg* segs, const int* begin, int nseg, int64_t blocks, int op_a,
                                  hipStream_t s);

@@ -127,15 +127,23 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
-// The pre-split kernel: prescaled fp16 and an even number of whole k-tiles
-// (A and B are packed by the plan's passes, whatever their layouts).
+// The pre-split kernel: prescaled fp16 or tf32 and an even number of whole
+// k-tiles (A is packed by the plan's pass whatever its layout; B is packed
+// too, except op_n B in the prescaled fp16 mode, read where it lies).
 const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t k,
                                         int64_t ldb, bool& direct_b) {
-  // op_n B is read in place when the tile's column offsets fit 32 bits, else packed
-  const auto& d = st->table.shgemm_ps[1][oc];
-  direct_b = ob == shgemm_amd::OP_N && (int64_t)d.bn * ldb * 2 < (int64_t(1) << 32);
-  const auto& e = st->table.shgemm_ps[direct_b ? 1 : 0][oc];
-  if (e.fn != nullptr && mode == shgemm_amd::MODE_FP16S && k >= 2 * e.bk && k % (2 * e.bk) == 0) return &e;
+  direct_b = false;
+  const shgemm_amd::KernelEntry* e = nullptr;
+  if (mode == shgemm_amd::MODE_FP16S) {
+    // op_n B is read in place when the tile's column offsets fit 32 bits, else packed
+    const auto& d = st->table.shgemm_ps[1][oc];
+    direct_b = ob == shgemm_amd::OP_N && (int64_t)d.bn * ldb * 2 < (int64_t(1) << 32);
+    e = &st->table.shgemm_ps[direct_b ? 1 : 0][oc];
+  } else if (mode == shgemm_amd::MODE_BF16) {
+    e = &st->table.shgemm_ps_bf16[oc];  // B's two BF16 parts are always packed
+  }
+  if (e != nullptr && e->fn != nullptr && k >= 2 * e->bk && k % (2 * e->bk) == 0) return e;
+  direct_b = false;
   return nullptr;
 }
 
@@ -408,6 +416,8 @@ void create(shgemmHandle_t& handle) {
     for (auto& a : st->table.shgemm_ps)
       for (auto& e : a)
         if (e.fn) set_lds_limit(e);
+    for (auto& e : st->table.shgemm_ps_bf16)
+      if (e.fn) set_lds_limit(e);
     check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
     shgemm_amd::init_static_scratch();
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");
@@ -595,13 +605,14 @@ detail::kernel_level shgemm_call(const shgemmHandle_t handle, const operation_t 
     plan.note = " (pre-split plan as the 256x256 tile: no allocation)";
   }
   if (plan.presplit) {
-    // A's packed hi and lo tiles (4 bytes per element, m rounded up to 256)
-    // and, for op_b = op_t, B's packed tiles (2 bytes per element, n rounded
-    // up to 256), in a handle buffer grown on demand; where the device cannot
-    // hold them, or the buffer would have to grow during graph capture, the
-    // call runs the register-split 256 x 256 tile (bit-identical).
-    const size_t bytes =
-        shgemm_amd::presplit_bytes(p.m, p.k) + (plan.direct_b ? 0 : shgemm_amd::pack_b_bytes(p.n, p.k));
+    // A's packed parts (prescaled fp16 hi and lo: 4 bytes per element; tf32
+    // three BF16 parts: 6; m rounded up to 256) and, unless op_n B is read in
+    // place, B's packed tiles (2 or 4 bytes per element, n rounded up to 256),
+    // in a handle buffer grown on demand; where the device cannot hold them,
+    // or the buffer would have to grow during graph capture, the call runs
+    // the register-split 256 x 256 tile (bit-identical).
+    const size_t bytes = shgemm_amd::presplit_bytes(p.m, p.k, mode) +
+                         (plan.direct_b ? 0 : shgemm_amd::pack_b_bytes(p.n, p.k, mode));
     if (!ensure_presplit(st, bytes, capturing())) {
       plan.presplit = false;
       plan.entry = nullptr;
@@ -640,14 +651,15 @@ detail::kernel_level shgemm_call(const shgemmHandle_t handle, const operation_t 
     p.row_exp = sc.row_exp;
   }
   if (plan.presplit) {
-    const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k);
+    const size_t a_bytes = shgemm_amd::presplit_bytes(p.m, p.k, mode);
     uint4* apack = reinterpret_cast<uint4*>(st->ps_buf);
     uint4* bpack = reinterpret_cast<uint4*>(reinterpret_cast<char*>(st->ps_buf) + a_bytes);
-    check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, st->row_exp, apack, stream),
+    check_hip(shgemm_amd::launch_presplit_a(a_ptr, (int64_t)lda, oa, p.m, p.k, mode, scaled ? st->row_exp : nullptr,
+                                            apack, stream),
               "pre-split pass");
     p.a = reinterpret_cast<const float*>(apack);  // the kernel reads the packed tiles
     if (!plan.direct_b) {
-      check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k,
+      check_hip(shgemm_amd::launch_pack_b(reinterpret_cast<const uint16_t*>(b_ptr), (int64_t)ldb, ob, p.n, p.k, mode,
                                           bpack, stream),
                 "B pack pass");
       p.b = bpack;
@@ -707,8 +719,9 @@ void reserve(shgemmHandle_t& handle, const std::size_t m, const std::size_t n, c
   if (m > 0 && n > 0 && k > 0 && !st->shared_device) {
     // the pre-split plan needs k % 64 == 0; reserve for k rounded up to that
     const int64_t k64 = ((int64_t)k + 63) / 64 * 64;
-    if (k64 <= INT_MAX) (void)ensure_presplit(st, shgemm_amd::presplit_bytes((int)m, (int)k64) +
-                                                      shgemm_amd::pack_b_bytes((int)n, (int)k64),
+    // sized for the larger (tf32) packing of A and B
+    if (k64 <= INT_MAX) (void)ensure_presplit(st, shgemm_amd::presplit_bytes((int)m, (int)k64, shgemm_amd::MODE_BF16) +
+                                                      shgemm_amd::pack_b_bytes((int)n, (int)k64, shgemm_amd::MODE_BF16),
                                               false);
   }
 }

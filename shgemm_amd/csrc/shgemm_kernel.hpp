@@ -471,6 +471,11 @@ struct ModeCfg {
   using P0W = P0;   // the small tile on whole-row exponents (prescaled mode only)
   using SK = TileCfg<128, 128, 32, 64, 64, 1, 2>;
   using P2 = TileCfg<256, 256, 32, 128, 128, 1, 1>;
+  // The tf32 pre-split ring: eight 64 x 128 wavefronts, two per SIMD, as in
+  // the prescaled fp16 mode (16384^3 per call 308 against 283 TFLOP/s for
+  // the four 128 x 128 wavefronts of P2, 8192^3 301 against 276; one box,
+  // interleaved).
+  using PS = TileCfg<256, 256, 32, 64, 128, 2, 2, true>;
   static constexpr bool kHasP2 = (MODE == MODE_BF16);
 };
 // The prescaled P2 keeps two k-tiles of loads in registers, splits tile kt+1
@@ -1478,6 +1483,157 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
   ktile(kt, cur, F{}, T{});
   ktile(kt + 1, cur == 2 ? 0 : cur + 1, F{}, F{});
   store_ctile<MODE_FP16S, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Pre-split kernel of the tf32 compute type: the ring of the fp16 pre-split
+// kernel on A's three BF16 parts and B's two, both packed in tile order
+// (launch_presplit_a / launch_pack_b with MODE_BF16: per 256-row block and
+// k-tile 48 KiB holding the a0, a1, a2 images; per 256-column block and
+// k-tile 32 KiB holding b0, b1).  A 256 x 256 x 32 stage is 80 KiB, so the
+// ring has two stages: tile kt+1 is issued into the other stage (it held
+// tile kt-1, whose reads completed before the last barrier) while tile kt is
+// multiplied, and the barrier after tile kt waits for it.  Each accumulator
+// takes, per 16-deep k-step in k order, the products a0*b0, a0*b1, a1*b0,
+// a1*b1, a2*b0 in that order, as the register-split kernels do (mma_kstep):
+// the results are bit-identical to them.  The host guarantees k % 64 == 0.
+// ---------------------------------------------------------------------------
+template <class CFG>
+struct PsBf16Layout {
+  static constexpr int A_CH = CFG::BM * CFG::KC, B_CH = CFG::BN * CFG::KC;
+  static constexpr int STAGE = 3 * A_CH + 2 * B_CH;  // 16-byte slots: a0, a1, a2, b0, b1
+  static constexpr int NSTAGE = 2;
+  static constexpr size_t lds_bytes() { return (size_t)NSTAGE * STAGE * 16; }
+};
+
+template <int FM, int FN>
+struct FragB16 {
+  bf16x8 a[3][FM], b[2][FN];
+};
+
+template <int OPC, class CFG>
+__global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps_bf16(const GemmParams p) {
+  constexpr int BM = CFG::BM, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
+  constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM, WAVES = NT / 64;
+  constexpr bool LANE_M = (OPC == OP_N);
+  static_assert(KC == 4, "two k-steps per k-tile");
+  using L = PsBf16Layout<CFG>;
+  constexpr int A_CH = L::A_CH, B_CH = L::B_CH, STAGE = L::STAGE;
+  static_assert(A_CH % NT == 0 && B_CH % NT == 0, "whole wavefront instructions per thread");
+  constexpr int QA = A_CH / NT, QB = B_CH / NT;  // instructions per thread per part
+  constexpr int PER = 3 * QA + 2 * QB;           // vector-memory instructions per thread per k-tile
+  using Frag = FragB16<FM, FN>;
+  constexpr int MF = FM * FN * 5;  // MFMAs per k-step
+  constexpr int NR = 3 * FM + 2 * FN;  // fragment reads per k-step
+  static_assert(PER <= NR && NR <= MF, "every load and fragment read has an MFMA beside it");
+  extern __shared__ uint4 lds[];
+
+  int tm, tn;
+  tile_coords(p, blockIdx.x, tm, tn);
+  const int m0 = tm * CFG::BM, n0 = tn * CFG::BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int lrow = lane & 31, lh = lane >> 5;
+  const int ktiles = p.k / BK;
+  const char* const apack =
+      uniform_ptr(reinterpret_cast<const char*>(p.a) + (int64_t)tm * ktiles * (3 * A_CH) * 16);
+  const char* const bpack =
+      uniform_ptr(reinterpret_cast<const char*>(p.b) + (int64_t)tn * ktiles * (2 * B_CH) * 16);
+
+  f32x16 acc[1][FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[0][i][j][e] = 0.0f;
+
+  // tile kt into stage st: lane l of instruction (q, wave) fills slot (q * WAVES + wave) * 64 + l of each part
+  auto issue = [&](int kt, int st) {
+    uint4* const dst = lds + st * STAGE;
+    const char* const at = apack + (int64_t)kt * (3 * A_CH * 16);
+    const char* const bt = bpack + (int64_t)kt * (2 * B_CH * 16);
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      const int blk = q * WAVES + wave;
+      const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
+#pragma unroll
+      for (int part = 0; part < 3; ++part) lds_dma16(at + part * A_CH * 16 + off, dst + part * A_CH + blk * 64);
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int blk = q * WAVES + wave;
+      const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
+#pragma unroll
+      for (int part = 0; part < 2; ++part)
+        lds_dma16(bt + part * B_CH * 16 + off, dst + 3 * A_CH + part * B_CH + blk * 64);
+    }
+  };
+  Frag f0, f1;
+  auto fload = [&](Frag& f, int st, int step) {
+    const uint4* const src = lds + st * STAGE;
+    const int c = 2 * step + lh;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int sl = lds_slot_h<KC>(wm * WM + i * 32 + lrow, c);
+#pragma unroll
+      for (int part = 0; part < 3; ++part) f.a[part][i] = *reinterpret_cast<const bf16x8*>(&src[part * A_CH + sl]);
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int sl = lds_slot_h<KC>(wn * WN + j * 32 + lrow, c);
+#pragma unroll
+      for (int part = 0; part < 2; ++part)
+        f.b[part][j] = *reinterpret_cast<const bf16x8*>(&src[3 * A_CH + part * B_CH + sl]);
+    }
+  };
+  // the five products one at a time over all accumulators (each accumulator
+  // still takes them in mma_kstep's order)
+  auto fmma = [&](const Frag& f) {
+    constexpr int PA[5] = {0, 0, 1, 1, 2}, PB[5] = {0, 1, 0, 1, 0};
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if constexpr (LANE_M) acc[0][i][j] = mfma_bf16(f.b[PB[t]][j], f.a[PA[t]][i], acc[0][i][j]);
+          else acc[0][i][j] = mfma_bf16(f.a[PA[t]][i], f.b[PB[t]][j], acc[0][i][j]);
+        }
+  };
+  // k-tile kt in stage cur; MORE: tile kt+1 exists
+  auto ktile = [&](int kt, int cur, auto more_c) {
+    constexpr bool MORE = decltype(more_c)::value;
+    if constexpr (MORE) issue(kt + 1, cur ^ 1);
+    fload(f1, cur, 1);
+    fmma(f0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      if (MORE && q < PER) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, MF - NR, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    lds_ring_barrier<0>();  // tile kt+1 has landed; every read of stage cur is done
+    if constexpr (MORE) fload(f0, cur ^ 1, 0);
+    fmma(f1);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, MF - NR, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  issue(0, 0);
+  lds_ring_barrier<0>();  // tile 0 landed
+  fload(f0, 0, 0);
+  int kt = 0;
+  for (; kt + 1 < ktiles; ++kt) ktile(kt, kt & 1, std::true_type{});
+  ktile(kt, kt & 1, std::false_type{});
+  store_ctile<MODE_BF16, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -361,16 +361,19 @@ def test_kernel_timing_counts_main_kernel_launches(h):
         sg.enable_kernel_timing(h, -1)
 
 
+@pytest.mark.parametrize("ct", ["fp16", "tf32"])
 @pytest.mark.parametrize("op_a,op_b,op_c", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1), (0, 1, 0)])
-def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c):
-    """Plan 8 (A split once per call and packed in tile order, B packed too,
-    the 256x256 tile then runs without a split) accumulates each C element
-    in the 256x256 tile's order: bit-identical to plan 5.  Cases cover both
-    B layouts (aligned and unaligned ldb), partial tiles in m and n, the
+def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c, ct):
+    """Plan 8 (A split once per call and packed in tile order -- prescaled
+    FP16 hi/lo, or tf32's three BF16 parts -- B read in place or packed, the
+    256x256 tile then runs without a split) accumulates each C element in the
+    256x256 tile's order: bit-identical to plan 5.  Cases cover both B
+    layouts (aligned and unaligned ldb), partial tiles in m and n, the
     shortest k (two k-tiles) and a padded lda / ldc."""
     import shgemm_amd as sg
 
-    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + op_c)
+    ctype = getattr(sg, ct)
+    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + op_c + (100 if ct == "tf32" else 0))
     for m, n, k in [(512, 768, 256), (1000, 600, 704), (257, 255, 128), (300, 517, 64), (64, 1100, 2048)]:
         lda, ldc = (m if op_a == 0 else k) + 3, (m if op_c == 0 else n) + 1
         ldb = (k if op_b == 0 else n) + (0 if op_a == 0 else 5)  # aligned and unaligned B
@@ -382,8 +385,8 @@ def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c):
         outs = []
         for plan in (5, 8):
             sg.force_plan(h, plan, 1)
-            assert sg.query_plan(h, op_a, op_b, m, n, k, sg.fp16, op_c)[0] == plan
-            sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, sg.fp16, op_c)
+            assert sg.query_plan(h, op_a, op_b, m, n, k, ctype, op_c)[0] == plan
+            sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, ctype, op_c)
             outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())
         sg.force_plan(h, -1, 1)
         assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (m, n, k)
@@ -398,7 +401,8 @@ def test_presplit_plan_falls_back_where_it_does_not_apply(h):
     assert sg.query_plan(h, 0, 0, 1024, 1024, 1000, sg.fp16)[0] == 5   # k not a multiple of 32
     assert sg.query_plan(h, 0, 0, 1024, 1024, 32, sg.fp16)[0] == 5     # one k-tile
     assert sg.query_plan(h, 0, 0, 1024, 1024, 96, sg.fp16)[0] == 5     # odd number of k-tiles
-    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.tf32)[0] == 5   # BF16 split
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.tf32)[0] == 8   # BF16 parts packed
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 96, sg.tf32)[0] == 5     # odd number of k-tiles
     sg.set_exponent_scaling(h, False)
     assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[0] == 1   # two-accumulator mode
     sg.set_exponent_scaling(h, True)

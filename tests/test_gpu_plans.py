@@ -392,6 +392,37 @@ def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c, ct):
         assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (m, n, k)
 
 
+@pytest.mark.parametrize("ct", ["fp16", "tf32"])
+def test_presplit_ring_repeats_p2_bits_on_random_shapes(h, ct):
+    """The pre-split kernels move every operand byte through an LDS ring whose
+    barriers wait only for the older k-tile: a load that landed late would
+    show as different bits on some call.  24 random shapes (k a multiple of
+    64, edges in m and n, both op_a), each run three times with plan 8 and
+    once with plan 5: every run bit-identical to plan 5."""
+    import shgemm_amd as sg
+
+    ctype = getattr(sg, ct)
+    rng = np.random.default_rng(1234 + (7 if ct == "tf32" else 0))
+    for _ in range(24):
+        m, n = (int(x) for x in rng.integers(200, 1400, 2))
+        k = 64 * int(rng.integers(2, 40))
+        op_a = int(rng.integers(0, 2))
+        lda = m if op_a == 0 else k
+        A = oracle.gen_uniform(rng, lda * (k if op_a == 0 else m))
+        B16 = oracle.gen_f16_normal(rng, k * n)
+        a_t, a_p = dev_buffer(A)
+        b_t, b_p = dev_buffer(B16)
+        c_t, c_p = dev_buffer(np.zeros(m * n, np.float32))
+        outs = []
+        for plan in (5, 8, 8, 8):
+            sg.force_plan(h, plan, 1)
+            sg.shgemm(h, op_a, 0, m, n, k, 1.0, a_p, lda, b_p, k, 0.0, c_p, m, ctype)
+            outs.append(read_buffer(c_t, 0, m * n).copy())
+        sg.force_plan(h, -1, 1)
+        for o in outs[1:]:
+            assert np.array_equal(outs[0].view(np.uint32), o.view(np.uint32)), (m, n, k, op_a)
+
+
 def test_presplit_plan_falls_back_where_it_does_not_apply(h):
     import shgemm_amd as sg
 

@@ -388,6 +388,13 @@ __device__ __forceinline__ void lds_dma16(const char* src, uint4* dst) {
   __builtin_amdgcn_global_load_lds(src, reinterpret_cast<uint32_t*>(dst), 16, 0, 0);
 }
 
+// Workgroup barrier for an LDS ring filled by global_load_lds: wait until at
+// most N of this thread's vector-memory operations are outstanding (the
+// older ones, whose LDS writes have then landed) and its LDS reads have
+// completed, then s_barrier.  No memory fence: a workgroup release fence
+// would also wait for the N newest loads, the ones meant to stay in flight.
+// The asm statement's memory clobber keeps the compiler from moving memory
+// accesses across the barrier.
 template <int N>
 __device__ __forceinline__ void lds_ring_barrier() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
@@ -1350,13 +1357,6 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ring(const Ge
 // Each C element accumulates in the order of the register-split P2 tile, so
 // the results are bit-identical to it.  The host guarantees k % 64 == 0.
 // ---------------------------------------------------------------------------
-// Workgroup barrier for an LDS ring filled by global_load_lds: wait until at
-// most N of this thread's vector-memory operations are outstanding (the
-// older ones, whose LDS writes have then landed) and its LDS reads have
-// completed, then s_barrier.  No memory fence: a workgroup release fence
-// would also wait for the N newest loads, the ones meant to stay in flight.
-// The asm statement's memory clobber keeps the compiler from moving memory
-// accesses across the barrier.
 template <class CFG>
 struct PsRingLayout {
   static constexpr int A_CH = CFG::BM * CFG::KC, B_CH = CFG::BN * CFG::KC;

@@ -6,8 +6,8 @@ Bars:
     the handle's split-K workspace, row-exponent scratch or pre-split
     buffer, give the same bits as the same calls run one at a time;
   * graph capture: after reserve(), captured calls (pre-split, 256x256 on
-    whole-row exponents, split-K) replay to the eager results on every
-    replay, with eager work between replays; without reserve a pre-split
+    whole-row exponents, split-K; fp16 and tf32) replay to the eager results
+    on every replay, with eager work between replays; without reserve a pre-split
     call runs as the 256x256 tile (bit-identical) and a call whose row
     scratch would have to grow raises before anything is enqueued;
   * plan 7 (deep prefetch) resolves to plan 1 where its per-lane byte
@@ -115,7 +115,8 @@ def _capture(torch, sg, h, fn):
     return graph
 
 
-def test_graph_capture_after_reserve(env):
+@pytest.mark.parametrize("ct", [0, 1])  # fp16, tf32 (its pre-split plan packs three BF16 parts of A, two of B)
+def test_graph_capture_after_reserve(env, ct):
     torch, sg, = env
     h = sg.create()
     sg.set_cuda_stream(h, torch.cuda.current_stream())
@@ -130,7 +131,7 @@ def test_graph_capture_after_reserve(env):
         def calls():
             for plan, sl, m, n, k, A, B, C in items:
                 sg.force_plan(h, plan, sl)
-                sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, FP16)
+                sg.shgemm(h, 0, 0, m, n, k, 1.0, A, m, B, k, 0.0, C, m, ct)
 
         calls()
         torch.cuda.synchronize()

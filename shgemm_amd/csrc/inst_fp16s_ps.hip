@@ -1,39 +1,45 @@
 // Pre-split prescaled fp16 kernels (A split once per call and packed in tile
 // order; B packed the same way, or for op_b = op_n read as it lies): C op_n
-// and op_t, on the 256 x 256 tile of eight wavefronts.
+// and op_t, on the 256 x 256 tile of eight wavefronts (one workgroup per CU)
+// and on 128 x 128 tiles of four (two per CU).
 #include "register_shgemm.hpp"
 namespace shgemm_amd {
-using CfgPS = ModeCfg<MODE_FP16S>::PS;
+template <class CFG, int OPC, bool DIRECT_B>
+static void put_ps(KernelEntry& e, int wgs_per_cu) {
+  static_assert(PsRingLayout<CFG>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CFG, DIRECT_B>);
+  e.lds = PsRingLayout<CFG>::lds_bytes();
+  e.bm = CFG::BM;
+  e.bn = CFG::BN;
+  e.bk = CFG::BK;
+  e.mode = MODE_FP16S;
+  e.threads = CFG::NT;
+  e.needs_row_exp = true;
+  e.wgs_per_cu = wgs_per_cu;
+}
 template <int OPC, bool DIRECT_B>
 static void put_ps(KernelTable& t) {
-  static_assert(PsRingLayout<CfgPS>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
-  KernelEntry& e = t.shgemm_ps[DIRECT_B][OPC];
-  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CfgPS, DIRECT_B>);
-  e.lds = PsRingLayout<CfgPS>::lds_bytes();
-  e.bm = CfgPS::BM;
-  e.bn = CfgPS::BN;
-  e.bk = CfgPS::BK;
-  e.mode = MODE_FP16S;
-  e.threads = CfgPS::NT;
-  e.needs_row_exp = true;
-  e.wgs_per_cu = 1;
+  put_ps<ModeCfg<MODE_FP16S>::PS, OPC, DIRECT_B>(t.shgemm_ps[DIRECT_B][OPC], 1);
+  put_ps<ModeCfg<MODE_FP16S>::PS128, OPC, DIRECT_B>(t.shgemm_ps128[DIRECT_B][OPC], 2);
 }
-// tf32 (BF16 parts): the same 256 x 256 tile of eight 64 x 128 wavefronts,
-// one workgroup per CU.
-using CfgPSB = ModeCfg<MODE_BF16>::PS;
+// tf32 (BF16 parts): the same tiles.
+template <class CFG, int OPC>
+static void put_ps_bf16(KernelEntry& e, int wgs_per_cu) {
+  static_assert(PsBf16Layout<CFG>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps_bf16<OPC, CFG>);
+  e.lds = PsBf16Layout<CFG>::lds_bytes();
+  e.bm = CFG::BM;
+  e.bn = CFG::BN;
+  e.bk = CFG::BK;
+  e.mode = MODE_BF16;
+  e.threads = CFG::NT;
+  e.needs_row_exp = false;
+  e.wgs_per_cu = wgs_per_cu;
+}
 template <int OPC>
 static void put_ps_bf16(KernelTable& t) {
-  static_assert(PsBf16Layout<CfgPSB>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
-  KernelEntry& e = t.shgemm_ps_bf16[OPC];
-  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps_bf16<OPC, CfgPSB>);
-  e.lds = PsBf16Layout<CfgPSB>::lds_bytes();
-  e.bm = CfgPSB::BM;
-  e.bn = CfgPSB::BN;
-  e.bk = CfgPSB::BK;
-  e.mode = MODE_BF16;
-  e.threads = CfgPSB::NT;
-  e.needs_row_exp = false;
-  e.wgs_per_cu = 1;
+  put_ps_bf16<ModeCfg<MODE_BF16>::PS, OPC>(t.shgemm_ps_bf16[OPC], 1);
+  put_ps_bf16<ModeCfg<MODE_BF16>::PS128, OPC>(t.shgemm_ps128_bf16[OPC], 2);
 }
 void register_shgemm_ps(KernelTable& t) {
   put_ps_bf16<OP_N>(t);

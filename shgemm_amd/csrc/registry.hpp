@@ -52,6 +52,9 @@ struct KernelTable {
   KernelEntry shgemm_ps[2][2];
   // pre-split tf32 kernel (A's three BF16 parts and B's two packed once per call): [op_c]
   KernelEntry shgemm_ps_bf16[2];
+  // the pre-split kernels on 128 x 128 tiles: [B in place][op_c] and tf32 [op_c]
+  KernelEntry shgemm_ps128[2][2];
+  KernelEntry shgemm_ps128_bf16[2];
   // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
   KernelEntry shgemm_p0w[2][2][2];
   // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]

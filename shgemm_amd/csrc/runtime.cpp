@@ -130,17 +130,22 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
 // The pre-split kernel: prescaled fp16 or tf32 and an even number of whole
 // k-tiles (A is packed by the plan's pass whatever its layout; B is packed
 // too, except op_n B in the prescaled fp16 mode, read where it lies).
-const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t k,
-                                        int64_t ldb, bool& direct_b) {
+const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t m,
+                                        int64_t n, int64_t k, int64_t ldb, bool& direct_b) {
   direct_b = false;
+  // 128 x 128 tiles where fewer than 3/4 of the CUs would get a 256 x 256 one
+  // (one box: 4096 x 2048 x 4096 128 tiles, 360 against 309 TFLOP/s for the
+  // 256 x 256 form; 3072^3 144 tiles, level; 3072 x 4096 x 4096 192 tiles,
+  // 421 against 437)
+  const bool small = ((m + 255) / 256) * ((n + 255) / 256) * 4 < std::max(1, st->num_cus) * 3;
   const shgemm_amd::KernelEntry* e = nullptr;
   if (mode == shgemm_amd::MODE_FP16S) {
     // op_n B is read in place when the tile's column offsets fit 32 bits, else packed
-    const auto& d = st->table.shgemm_ps[1][oc];
-    direct_b = ob == shgemm_amd::OP_N && (int64_t)d.bn * ldb * 2 < (int64_t(1) << 32);
-    e = &st->table.shgemm_ps[direct_b ? 1 : 0][oc];
+    const auto& tab = small ? st->table.shgemm_ps128 : st->table.shgemm_ps;
+    direct_b = ob == shgemm_amd::OP_N && (int64_t)tab[1][oc].bn * ldb * 2 < (int64_t(1) << 32);
+    e = &tab[direct_b ? 1 : 0][oc];
   } else if (mode == shgemm_amd::MODE_BF16) {
-    e = &st->table.shgemm_ps_bf16[oc];  // B's two BF16 parts are always packed
+    e = small ? &st->table.shgemm_ps128_bf16[oc] : &st->table.shgemm_ps_bf16[oc];  // B always packed
   }
   if (e != nullptr && e->fn != nullptr && k >= 2 * e->bk && k % (2 * e->bk) == 0) return e;
   direct_b = false;
@@ -173,7 +178,7 @@ Plan resolve_plan(const detail::handle_state* st, const shgemm_amd::KernelEntry 
     requested = tab[shgemm_amd::V_P2].fn != nullptr ? shgemm_amd::V_P2 : shgemm_amd::V_P1;
   if (requested == plan_variant_ps) {
     bool direct_b = false;
-    pl.entry = ps_entry(st, mode, ob, oc, k, ldb, direct_b);
+    pl.entry = ps_entry(st, mode, ob, oc, m, n, k, ldb, direct_b);
     if (pl.entry != nullptr) {
       pl.variant = shgemm_amd::V_P2;
       pl.presplit = true;
@@ -417,6 +422,11 @@ void create(shgemmHandle_t& handle) {
       for (auto& e : a)
         if (e.fn) set_lds_limit(e);
     for (auto& e : st->table.shgemm_ps_bf16)
+      if (e.fn) set_lds_limit(e);
+    for (auto& a : st->table.shgemm_ps128)
+      for (auto& e : a)
+        if (e.fn) set_lds_limit(e);
+    for (auto& e : st->table.shgemm_ps128_bf16)
       if (e.fn) set_lds_limit(e);
     check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
     shgemm_amd::init_static_scratch();

@@ -483,6 +483,8 @@ struct ModeCfg {
   // the four 128 x 128 wavefronts of P2, 8192^3 301 against 276; one box,
   // interleaved).
   using PS = TileCfg<256, 256, 32, 64, 128, 2, 2, true>;
+  // the same ring on 128 x 128 tiles (shapes with fewer 256 x 256 tiles than CUs)
+  using PS128 = TileCfg<128, 128, 32, 64, 64, 2, 2, true>;
   static constexpr bool kHasP2 = (MODE == MODE_BF16);
 };
 // The prescaled P2 keeps two k-tiles of loads in registers, splits tile kt+1
@@ -524,6 +526,8 @@ struct ModeCfg<MODE_FP16S> {
   // and IL describe the register-split P2 form; the pre-split kernel stages
   // its loads in an LDS ring.)
   using PS = TileCfg<256, 256, 32, 64, 128, 2, 2, true>;
+  // the same ring on 128 x 128 tiles (shapes with fewer 256 x 256 tiles than CUs)
+  using PS128 = TileCfg<128, 128, 32, 64, 64, 2, 2, true>;
   static constexpr bool kHasP2 = true;
 };
 
@@ -1357,6 +1361,13 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ring(const Ge
 // Each C element accumulates in the order of the register-split P2 tile, so
 // the results are bit-identical to it.  The host guarantees k % 64 == 0.
 // ---------------------------------------------------------------------------
+// Rows (columns) of one packed block of the pre-split operands
+// (launch_presplit_a, launch_pack_b).  A tile of BM <= 256 rows reads rows
+// m0 % 256 .. + BM of block m0 / 256: the LDS-image slot of row r is
+// r * KC + (chunk ^ key(r)) with a key that depends on r % 16 only, so those
+// rows are already the image of a BM-row tile.
+constexpr int kPsBlock = 256;
+
 template <class CFG>
 struct PsRingLayout {
   static constexpr int A_CH = CFG::BM * CFG::KC, B_CH = CFG::BN * CFG::KC;
@@ -1373,6 +1384,8 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
   static_assert(KC == 4, "two k-steps per k-tile");
   using L = PsRingLayout<CFG>;
   constexpr int A_CH = L::A_CH, B_CH = L::B_CH, STAGE = L::STAGE;
+  constexpr int A_PL = kPsBlock * KC, B_PL = kPsBlock * KC;  // slots of one packed plane per k-tile
+  static_assert(kPsBlock % CFG::BM == 0 && kPsBlock % CFG::BN == 0, "tiles inside the packed blocks");
   static_assert(A_CH % NT == 0 && B_CH % NT == 0, "whole wavefront instructions per thread");
   constexpr int QA = A_CH / NT, QB = B_CH / NT;  // instructions per thread: each A half, B
   constexpr int PER = 2 * QA + QB;               // vector-memory instructions per thread per k-tile
@@ -1390,14 +1403,16 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
   const int ktiles = p.k / BK;
-  // this block row's packed k-tiles (2 * A_CH slots each: hi, then lo)
-  const char* const apack =
-      uniform_ptr(reinterpret_cast<const char*>(p.a) + (int64_t)tm * ktiles * (2 * A_CH) * 16);
-  // B: this block column's packed k-tiles (B_CH slots each), or column n0 of the caller's B
+  // A's packed k-tiles of this tile's rows: the tile is rows m0 % 256 .. + BM
+  // of packed block m0 / 256 (2 * A_PL slots per k-tile: hi, then lo)
+  const char* const apack = uniform_ptr(reinterpret_cast<const char*>(p.a) +
+                                        ((int64_t)(m0 / kPsBlock) * ktiles * (2 * A_PL) + (m0 % kPsBlock) * KC) * 16);
+  // B: this tile's columns of the packed k-tiles (B_PL slots each), or column n0 of the caller's B
   const char* const bbase =
       DIRECT_B ? uniform_ptr(reinterpret_cast<const char*>(static_cast<const uint16_t*>(p.b) + (int64_t)n0 * p.ldb))
-               : uniform_ptr(reinterpret_cast<const char*>(p.b) + (int64_t)tn * ktiles * B_CH * 16);
-  constexpr int64_t B_STEP = DIRECT_B ? BK * 2 : B_CH * 16;  // bytes from one k-tile of B to the next
+               : uniform_ptr(reinterpret_cast<const char*>(p.b) +
+                             ((int64_t)(n0 / kPsBlock) * ktiles * B_PL + (n0 % kPsBlock) * KC) * 16);
+  constexpr int64_t B_STEP = DIRECT_B ? BK * 2 : B_PL * 16;  // bytes from one k-tile of B to the next
   uint32_t boff[QB];
 #pragma unroll
   for (int q = 0; q < QB; ++q) {
@@ -1421,13 +1436,13 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
   // tile kt into stage st: lane l of instruction (q, wave) fills slot (q * WAVES + wave) * 64 + l
   auto issue = [&](int kt, int st) {
     uint4* const dst = lds + st * STAGE;
-    const char* const at = apack + (int64_t)kt * (2 * A_CH * 16);
+    const char* const at = apack + (int64_t)kt * (2 * A_PL * 16);
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int blk = q * WAVES + wave;
       const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
       lds_dma16(at + off, dst + blk * 64);
-      lds_dma16(at + A_CH * 16 + off, dst + A_CH + blk * 64);
+      lds_dma16(at + A_PL * 16 + off, dst + A_CH + blk * 64);
     }
     const char* const bt = bbase + (int64_t)kt * B_STEP;
 #pragma unroll
@@ -1519,6 +1534,8 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps_bf16(const
   static_assert(KC == 4, "two k-steps per k-tile");
   using L = PsBf16Layout<CFG>;
   constexpr int A_CH = L::A_CH, B_CH = L::B_CH, STAGE = L::STAGE;
+  constexpr int A_PL = kPsBlock * KC, B_PL = kPsBlock * KC;  // slots of one packed plane per k-tile
+  static_assert(kPsBlock % CFG::BM == 0 && kPsBlock % CFG::BN == 0, "tiles inside the packed blocks");
   static_assert(A_CH % NT == 0 && B_CH % NT == 0, "whole wavefront instructions per thread");
   constexpr int QA = A_CH / NT, QB = B_CH / NT;  // instructions per thread per part
   constexpr int PER = 3 * QA + 2 * QB;           // vector-memory instructions per thread per k-tile
@@ -1536,10 +1553,11 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps_bf16(const
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
   const int lrow = lane & 31, lh = lane >> 5;
   const int ktiles = p.k / BK;
-  const char* const apack =
-      uniform_ptr(reinterpret_cast<const char*>(p.a) + (int64_t)tm * ktiles * (3 * A_CH) * 16);
-  const char* const bpack =
-      uniform_ptr(reinterpret_cast<const char*>(p.b) + (int64_t)tn * ktiles * (2 * B_CH) * 16);
+  // this tile's rows / columns of the packed blocks (3 * A_PL, 2 * B_PL slots per k-tile)
+  const char* const apack = uniform_ptr(reinterpret_cast<const char*>(p.a) +
+                                        ((int64_t)(m0 / kPsBlock) * ktiles * (3 * A_PL) + (m0 % kPsBlock) * KC) * 16);
+  const char* const bpack = uniform_ptr(reinterpret_cast<const char*>(p.b) +
+                                        ((int64_t)(n0 / kPsBlock) * ktiles * (2 * B_PL) + (n0 % kPsBlock) * KC) * 16);
 
   f32x16 acc[1][FM][FN];
 #pragma unroll
@@ -1552,14 +1570,14 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps_bf16(const
   // tile kt into stage st: lane l of instruction (q, wave) fills slot (q * WAVES + wave) * 64 + l of each part
   auto issue = [&](int kt, int st) {
     uint4* const dst = lds + st * STAGE;
-    const char* const at = apack + (int64_t)kt * (3 * A_CH * 16);
-    const char* const bt = bpack + (int64_t)kt * (2 * B_CH * 16);
+    const char* const at = apack + (int64_t)kt * (3 * A_PL * 16);
+    const char* const bt = bpack + (int64_t)kt * (2 * B_PL * 16);
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int blk = q * WAVES + wave;
       const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
 #pragma unroll
-      for (int part = 0; part < 3; ++part) lds_dma16(at + part * A_CH * 16 + off, dst + part * A_CH + blk * 64);
+      for (int part = 0; part < 3; ++part) lds_dma16(at + part * A_PL * 16 + off, dst + part * A_CH + blk * 64);
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -1567,7 +1585,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps_bf16(const
       const uint32_t off = (uint32_t)(blk * 64 + lane) * 16u;
 #pragma unroll
       for (int part = 0; part < 2; ++part)
-        lds_dma16(bt + part * B_CH * 16 + off, dst + 3 * A_CH + part * B_CH + blk * 64);
+        lds_dma16(bt + part * B_PL * 16 + off, dst + 3 * A_CH + part * B_CH + blk * 64);
     }
   };
   Frag f0, f1;

@@ -14,10 +14,11 @@ Bars:
     offsets would not fit 32 bits (op_b = N with ldb = k >= 2^23, op_t A
     with lda = k >= 2^23), and stays plan 7 below that (query only: nothing
     of that size is launched);
-  * plan 8 (pre-split) reads op_n B where it lies while the 256-column
-    tile's byte offsets fit 32 bits (256 * ldb * 2 < 2^32) and packs it
-    otherwise; on both sides of that limit it gives plan 5's bits (B of
-    three columns at ldb = 2^23 - 8 and 2^23: a 16 MiB buffer);
+  * plan 8 (pre-split) reads op_n B where it lies while the tile's column
+    byte offsets fit 32 bits (BN * ldb * 2 < 2^32; this shape runs the
+    128 x 128 form) and packs it otherwise; on both sides of that limit it
+    gives plan 5's bits (B of three columns at ldb = 2^24 - 8 and 2^24: a
+    64 MiB buffer);
   * set_debug_mode prints the reference's debug line for each call.
 """
 from __future__ import annotations
@@ -228,7 +229,7 @@ def test_debug_mode_prints_call_line(env, capfd):
         sg.destroy(h)
 
 
-@pytest.mark.parametrize("ldb", [(1 << 23) - 8, 1 << 23])
+@pytest.mark.parametrize("ldb", [(1 << 24) - 8, 1 << 24])
 def test_presplit_reads_b_in_place_below_the_offset_limit(env, capfd, ldb):
     torch, sg = env
     h = sg.create()
@@ -251,7 +252,8 @@ def test_presplit_reads_b_in_place_below_the_offset_limit(env, capfd, ldb):
             out[plan] = C.cpu().numpy()
         lines = [l for l in capfd.readouterr().out.splitlines() if l.startswith("[shape=")]
         assert len(lines) == 2, lines
-        in_place = 256 * ldb * 2 < (1 << 32)
+        assert "tile=128x128x32" in lines[0], lines[0]
+        in_place = 128 * ldb * 2 < (1 << 32)
         assert ("pre-split A, B in place" in lines[0]) == in_place, lines[0]
         assert "pre-split" in lines[0], lines[0]
         assert np.array_equal(out[PLAN_PS].view(np.uint32), out[PLAN_P2].view(np.uint32))

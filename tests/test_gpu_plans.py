@@ -369,12 +369,15 @@ def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c, ct):
     256x256 tile then runs without a split) accumulates each C element in the
     256x256 tile's order: bit-identical to plan 5.  Cases cover both B
     layouts (aligned and unaligned ldb), partial tiles in m and n, the
-    shortest k (two k-tiles) and a padded lda / ldc."""
+    shortest k (two k-tiles) and a padded lda / ldc; the smaller shapes run
+    the 128x128 form (fewer 256x256 tiles than CUs), 4096 x 4096 the
+    256x256 one."""
     import shgemm_amd as sg
 
     ctype = getattr(sg, ct)
     rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + op_c + (100 if ct == "tf32" else 0))
-    for m, n, k in [(512, 768, 256), (1000, 600, 704), (257, 255, 128), (300, 517, 64), (64, 1100, 2048)]:
+    for m, n, k in [(512, 768, 256), (1000, 600, 704), (257, 255, 128), (300, 517, 64), (64, 1100, 2048),
+                    (4096, 4096, 128)]:
         lda, ldc = (m if op_a == 0 else k) + 3, (m if op_c == 0 else n) + 1
         ldb = (k if op_b == 0 else n) + (0 if op_a == 0 else 5)  # aligned and unaligned B
         A = oracle.gen_uniform(rng, lda * (k if op_a == 0 else m))

@@ -418,14 +418,17 @@ hipError_t launch_grouped_reduce(const ReduceSeg* segs, const int* begin, int ns
 // Caller: k % 32 == 0.  One workgroup of 256 threads per tile.
 // BF16 (the tf32 compute type): the three BF16 parts a0, a1, a2 of each
 // element (split_bf16x3, as the register-split kernels make them), no row
-// exponents; a tile is 48 KiB, the a0, a1 then a2 images.
+// exponents; a tile is 48 KiB, the a0, a1 then a2 images.  MODE_FP16
+// (prescaling off): FP16 hi and the 2^11-scaled residual (split_f16), no
+// row exponents.
 // ---------------------------------------------------------------------------
 constexpr int kPackRows = 256, kPackK = 32, kPackSlots = kPackRows * kPackK / 8;
 
-template <bool KCONT, bool BF16>
+template <bool KCONT, int MODE>
 __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restrict__ a, int64_t lda, int m, int k,
                                                             int vec, const int* __restrict__ row_exp,
                                                             uint4* __restrict__ out) {
+  constexpr bool BF16 = MODE == MODE_BF16;
   constexpr int PLANES = BF16 ? 3 : 2;
   __shared__ float t[KCONT ? 1 : kPackK][kPackRows + 1];
   const int mb = (m + kPackRows - 1) / kPackRows, kt_n = k / kPackK;
@@ -482,6 +485,17 @@ __global__ void __launch_bounds__(256) presplit_pack_kernel(const float* __restr
       tile[s] = __builtin_bit_cast(uint4, x0);
       tile[kPackSlots + s] = __builtin_bit_cast(uint4, x1);
       tile[2 * kPackSlots + s] = __builtin_bit_cast(uint4, x2);
+    } else if constexpr (MODE == MODE_FP16) {
+      f16x8 h, l;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        _Float16 hh, ll;
+        split_f16(v[e], hh, ll);
+        h[e] = hh;
+        l[e] = ll;
+      }
+      tile[s] = __builtin_bit_cast(uint4, h);
+      tile[kPackSlots + s] = __builtin_bit_cast(uint4, l);
     } else {
       const float sc = r0 + r < m ? __builtin_ldexpf(1.0f, -row_exp[r0 + r]) : 1.0f;
       uint4 h, l;
@@ -588,9 +602,13 @@ hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k
   const int64_t blocks = (int64_t)((m + kPackRows - 1) / kPackRows) * (k / kPackK);
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
   const int vec = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (lda % 4 == 0);
-  const bool bf16 = mode == MODE_BF16;
-  auto fn = op_a == OP_T ? (bf16 ? presplit_pack_kernel<true, true> : presplit_pack_kernel<true, false>)
-                         : (bf16 ? presplit_pack_kernel<false, true> : presplit_pack_kernel<false, false>);
+  auto pick = [&](auto kcont_c) {
+    constexpr bool KC = decltype(kcont_c)::value;
+    return mode == MODE_BF16 ? presplit_pack_kernel<KC, MODE_BF16>
+           : mode == MODE_FP16 ? presplit_pack_kernel<KC, MODE_FP16>
+                               : presplit_pack_kernel<KC, MODE_FP16S>;
+  };
+  auto fn = op_a == OP_T ? pick(std::true_type{}) : pick(std::false_type{});
   hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, s, a, lda, m, k, vec, row_exp, out);
   return hipGetLastError();
 }

@@ -4,23 +4,26 @@
 // and on 128 x 128 tiles of four (two per CU).
 #include "register_shgemm.hpp"
 namespace shgemm_amd {
-template <class CFG, int OPC, bool DIRECT_B>
+template <class CFG, int OPC, bool DIRECT_B, int MODE>
 static void put_ps(KernelEntry& e, int wgs_per_cu) {
   static_assert(PsRingLayout<CFG>::lds_bytes() <= kMaxLdsBytes, "ring exceeds the LDS of a CU");
-  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CFG, DIRECT_B>);
+  e.fn = reinterpret_cast<const void*>(&shgemm_kernel_ps<OPC, CFG, DIRECT_B, MODE>);
   e.lds = PsRingLayout<CFG>::lds_bytes();
   e.bm = CFG::BM;
   e.bn = CFG::BN;
   e.bk = CFG::BK;
-  e.mode = MODE_FP16S;
+  e.mode = MODE;
   e.threads = CFG::NT;
-  e.needs_row_exp = true;
+  e.needs_row_exp = MODE == MODE_FP16S;
   e.wgs_per_cu = wgs_per_cu;
 }
 template <int OPC, bool DIRECT_B>
 static void put_ps(KernelTable& t) {
-  put_ps<ModeCfg<MODE_FP16S>::PS, OPC, DIRECT_B>(t.shgemm_ps[DIRECT_B][OPC], 1);
-  put_ps<ModeCfg<MODE_FP16S>::PS128, OPC, DIRECT_B>(t.shgemm_ps128[DIRECT_B][OPC], 2);
+  put_ps<ModeCfg<MODE_FP16S>::PS, OPC, DIRECT_B, MODE_FP16S>(t.shgemm_ps[DIRECT_B][OPC], 1);
+  put_ps<ModeCfg<MODE_FP16S>::PS128, OPC, DIRECT_B, MODE_FP16S>(t.shgemm_ps128[DIRECT_B][OPC], 2);
+  // prescaling off: the 128 x 128 form only (the 64 x 128 wavefronts would
+  // need 256 accumulator registers)
+  put_ps<ModeCfg<MODE_FP16S>::PS128, OPC, DIRECT_B, MODE_FP16>(t.shgemm_ps128_fp16[DIRECT_B][OPC], 2);
 }
 // tf32 (BF16 parts): the same tiles.
 template <class CFG, int OPC>

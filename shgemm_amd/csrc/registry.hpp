@@ -55,6 +55,8 @@ struct KernelTable {
   // the pre-split kernels on 128 x 128 tiles: [B in place][op_c] and tf32 [op_c]
   KernelEntry shgemm_ps128[2][2];
   KernelEntry shgemm_ps128_bf16[2];
+  // the 128 x 128 pre-split kernel with prescaling off (two accumulators): [B in place][op_c]
+  KernelEntry shgemm_ps128_fp16[2][2];
   // prescaled fp16 small tile on whole-row exponents: [op_a][op_b][op_c]
   KernelEntry shgemm_p0w[2][2][2];
   // prescaled fp16 deep k-sliced 128 x 128 x 64 tile (plan 10): [op_a][op_b][op_c]
@@ -145,8 +147,9 @@ hipError_t launch_row_exponents(const float* a, int64_t lda, int op_a, int m, in
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // A split in the pre-split plan's tile order (256-row blocks x 32-deep
 // k-tiles): prescaled FP16 hi / residual lo (MODE_FP16S, 32 KiB per tile,
-// row_exp from the row pass) or three BF16 parts (MODE_BF16, 48 KiB per
-// tile, row_exp unused); k % 32 == 0.  The output needs
+// row_exp from the row pass), FP16 hi / 2^11-scaled residual (MODE_FP16,
+// row_exp unused) or three BF16 parts (MODE_BF16, 48 KiB per tile, row_exp
+// unused); k % 32 == 0.  The output needs
 // presplit_bytes(m, k, mode) bytes.
 size_t presplit_bytes(int m, int k, int mode);
 hipError_t launch_presplit_a(const float* a, int64_t lda, int op_a, int m, int k, int mode, const int* row_exp,

@@ -127,9 +127,9 @@ const shgemm_amd::KernelEntry* bd_entry(const detail::handle_state* st, int mode
   return nullptr;
 }
 
-// The pre-split kernel: prescaled fp16 or tf32 and an even number of whole
-// k-tiles (A is packed by the plan's pass whatever its layout; B is packed
-// too, except op_n B in the prescaled fp16 mode, read where it lies).
+// The pre-split kernel: every compute mode, an even number of whole k-tiles
+// (A is packed by the plan's pass whatever its layout; B is packed too,
+// except op_n B in the fp16 modes, read where it lies).
 const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode, int ob, int oc, int64_t m,
                                         int64_t n, int64_t k, int64_t ldb, bool& direct_b) {
   direct_b = false;
@@ -146,6 +146,11 @@ const shgemm_amd::KernelEntry* ps_entry(const detail::handle_state* st, int mode
     e = &tab[direct_b ? 1 : 0][oc];
   } else if (mode == shgemm_amd::MODE_BF16) {
     e = small ? &st->table.shgemm_ps128_bf16[oc] : &st->table.shgemm_ps_bf16[oc];  // B always packed
+  } else if (mode == shgemm_amd::MODE_FP16) {
+    // prescaling off: the 128 x 128 form at every size
+    const auto& tab = st->table.shgemm_ps128_fp16;
+    direct_b = ob == shgemm_amd::OP_N && (int64_t)tab[1][oc].bn * ldb * 2 < (int64_t(1) << 32);
+    e = &tab[direct_b ? 1 : 0][oc];
   }
   if (e != nullptr && e->fn != nullptr && k >= 2 * e->bk && k % (2 * e->bk) == 0) return e;
   direct_b = false;
@@ -428,6 +433,9 @@ void create(shgemmHandle_t& handle) {
         if (e.fn) set_lds_limit(e);
     for (auto& e : st->table.shgemm_ps128_bf16)
       if (e.fn) set_lds_limit(e);
+    for (auto& a : st->table.shgemm_ps128_fp16)
+      for (auto& e : a)
+        if (e.fn) set_lds_limit(e);
     check_hip(hipEventCreateWithFlags(&st->scratch_ev, hipEventDisableTiming), "hipEventCreate(scratch order)");
     shgemm_amd::init_static_scratch();
     check_hip(hipMalloc(&st->ws, kWorkspaceBytes), "hipMalloc(workspace)");

@@ -1376,11 +1376,15 @@ struct PsRingLayout {
   static constexpr size_t lds_bytes() { return (size_t)NSTAGE * STAGE * 16; }
 };
 
-template <int OPC, class CFG, bool DIRECT_B>
+template <int OPC, class CFG, bool DIRECT_B, int MODE = MODE_FP16S>
 __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const GemmParams p) {
   constexpr int BM = CFG::BM, BK = CFG::BK, WM = CFG::WM, WN = CFG::WN, KC = CFG::KC, NT = CFG::NT;
   constexpr int FM = WM / 32, FN = WN / 32, WAVES_M = BM / WM, WAVES = NT / 64;
   constexpr bool LANE_M = (OPC == OP_N);
+  // MODE_FP16 (prescaling off): hi and 2^11-scaled lo products in two
+  // accumulators, as the register-split kernels of that mode keep them
+  static_assert(MODE == MODE_FP16S || MODE == MODE_FP16, "FP16 parts");
+  constexpr int NACC = MODE == MODE_FP16 ? 2 : 1;
   static_assert(KC == 4, "two k-steps per k-tile");
   using L = PsRingLayout<CFG>;
   constexpr int A_CH = L::A_CH, B_CH = L::B_CH, STAGE = L::STAGE;
@@ -1425,13 +1429,29 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
     }
   }
 
-  f32x16 acc[1][FM][FN];
+  f32x16 acc[NACC][FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int h = 0; h < NACC; ++h)
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) acc[0][i][j][e] = 0.0f;
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[h][i][j][e] = 0.0f;
+  // all hi products of a k-step, then all lo products (into acc[NACC - 1])
+  auto fmma = [&](const FragF16<FM, FN>& f) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const f16x8& a = h == 0 ? f.ah[i] : f.al[i];
+          f32x16& c = acc[h == 0 ? 0 : NACC - 1][i][j];
+          if constexpr (LANE_M) c = mfma_f16(f.bb[j], a, c);
+          else c = mfma_f16(a, f.bb[j], c);
+        }
+  };
 
   // tile kt into stage st: lane l of instruction (q, wave) fills slot (q * WAVES + wave) * 64 + l
   auto issue = [&](int kt, int st) {
@@ -1462,7 +1482,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
     const int nxt = cur == 2 ? 0 : cur + 1, far = cur == 0 ? 2 : cur - 1;
     if constexpr (MORE) issue(kt + 2, far);
     fload(f1, cur, 1);
-    frag_mma<LANE_M>(acc, f0);
+    fmma(f0);
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
@@ -1475,7 +1495,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
     if constexpr (MORE) lds_ring_barrier<PER>();
     else lds_ring_barrier<0>();
     if constexpr (NEXT) fload(f0, nxt, 0);
-    frag_mma<LANE_M>(acc, f1);
+    fmma(f1);
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
@@ -1497,7 +1517,7 @@ __global__ void __launch_bounds__(CFG::NT, CFG::OCC) shgemm_kernel_ps(const Gemm
   }
   ktile(kt, cur, F{}, T{});
   ktile(kt + 1, cur == 2 ? 0 : cur + 1, F{}, F{});
-  store_ctile<MODE_FP16S, LANE_M, false, 1, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
+  store_ctile<MODE, LANE_M, false, NACC, FM, FN, WM, WN>(acc, p, m0, n0, wm, wn, lrow, lh, nullptr);
 }
 
 // ---------------------------------------------------------------------------

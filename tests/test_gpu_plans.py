@@ -361,7 +361,7 @@ def test_kernel_timing_counts_main_kernel_launches(h):
         sg.enable_kernel_timing(h, -1)
 
 
-@pytest.mark.parametrize("ct", ["fp16", "tf32"])
+@pytest.mark.parametrize("ct", ["fp16", "tf32", "classic"])
 @pytest.mark.parametrize("op_a,op_b,op_c", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1), (0, 1, 0)])
 def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c, ct):
     """Plan 8 (A split once per call and packed in tile order -- prescaled
@@ -371,41 +371,60 @@ def test_presplit_plan_matches_p2_bitwise(h, op_a, op_b, op_c, ct):
     layouts (aligned and unaligned ldb), partial tiles in m and n, the
     shortest k (two k-tiles) and a padded lda / ldc; the smaller shapes run
     the 128x128 form (fewer 256x256 tiles than CUs), 4096 x 4096 the
-    256x256 one."""
+    256x256 one.  With prescaling off ("classic": FP16 hi and 2^11-scaled lo
+    in two accumulators) plan 8 runs the 128x128 form at every size and
+    matches plan 1, whose 64x64 wavefronts take the same products in the same
+    order."""
     import shgemm_amd as sg
 
-    ctype = getattr(sg, ct)
-    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + op_c + (100 if ct == "tf32" else 0))
-    for m, n, k in [(512, 768, 256), (1000, 600, 704), (257, 255, 128), (300, 517, 64), (64, 1100, 2048),
-                    (4096, 4096, 128)]:
-        lda, ldc = (m if op_a == 0 else k) + 3, (m if op_c == 0 else n) + 1
-        ldb = (k if op_b == 0 else n) + (0 if op_a == 0 else 5)  # aligned and unaligned B
-        A = oracle.gen_uniform(rng, lda * (k if op_a == 0 else m))
-        B16 = oracle.gen_f16_normal(rng, ldb * (n if op_b == 0 else k))
-        a_t, a_p = dev_buffer(A)
-        b_t, b_p = dev_buffer(B16)
-        c_t, c_p = dev_buffer(np.zeros(ldc * (n if op_c == 0 else m), np.float32))
-        outs = []
-        for plan in (5, 8):
-            sg.force_plan(h, plan, 1)
-            assert sg.query_plan(h, op_a, op_b, m, n, k, ctype, op_c)[0] == plan
-            sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, ctype, op_c)
-            outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())
+    ctype = sg.fp16 if ct == "classic" else getattr(sg, ct)
+    ref_plan = 1 if ct == "classic" else 5
+    rng = np.random.default_rng(40 + 4 * op_a + 2 * op_b + op_c + {"fp16": 0, "tf32": 100, "classic": 200}[ct])
+    sg.set_exponent_scaling(h, ct != "classic")
+    try:
+        for m, n, k in [(512, 768, 256), (1000, 600, 704), (257, 255, 128), (300, 517, 64), (64, 1100, 2048),
+                        (4096, 4096, 128)]:
+            lda, ldc = (m if op_a == 0 else k) + 3, (m if op_c == 0 else n) + 1
+            ldb = (k if op_b == 0 else n) + (0 if op_a == 0 else 5)  # aligned and unaligned B
+            A = oracle.gen_uniform(rng, lda * (k if op_a == 0 else m))
+            B16 = oracle.gen_f16_normal(rng, ldb * (n if op_b == 0 else k))
+            a_t, a_p = dev_buffer(A)
+            b_t, b_p = dev_buffer(B16)
+            c_t, c_p = dev_buffer(np.zeros(ldc * (n if op_c == 0 else m), np.float32))
+            outs = []
+            for plan in (ref_plan, 8):
+                sg.force_plan(h, plan, 1)
+                assert sg.query_plan(h, op_a, op_b, m, n, k, ctype, op_c)[0] == plan
+                sg.shgemm(h, op_a, op_b, m, n, k, 1.0, a_p, lda, b_p, ldb, 0.0, c_p, ldc, ctype, op_c)
+                outs.append(read_buffer(c_t, 0, c_t.numel() - 8).copy())
+            sg.force_plan(h, -1, 1)
+            assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (m, n, k)
+    finally:
         sg.force_plan(h, -1, 1)
-        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (m, n, k)
+        sg.set_exponent_scaling(h, True)
 
 
-@pytest.mark.parametrize("ct", ["fp16", "tf32"])
+@pytest.mark.parametrize("ct", ["fp16", "tf32", "classic"])
 def test_presplit_ring_repeats_p2_bits_on_random_shapes(h, ct):
     """The pre-split kernels move every operand byte through an LDS ring whose
     barriers wait only for the older k-tile: a load that landed late would
     show as different bits on some call.  24 random shapes (k a multiple of
     64, edges in m and n, both op_a), each run three times with plan 8 and
-    once with plan 5: every run bit-identical to plan 5."""
+    once with plan 5 (with prescaling off, "classic", plan 5 runs as plan 1):
+    every run bit-identical to plan 5."""
     import shgemm_amd as sg
 
-    ctype = getattr(sg, ct)
-    rng = np.random.default_rng(1234 + (7 if ct == "tf32" else 0))
+    ctype = sg.fp16 if ct == "classic" else getattr(sg, ct)
+    rng = np.random.default_rng(1234 + {"fp16": 0, "tf32": 7, "classic": 11}[ct])
+    sg.set_exponent_scaling(h, ct != "classic")
+    try:
+        _random_shape_runs(sg, h, ctype, rng)
+    finally:
+        sg.force_plan(h, -1, 1)
+        sg.set_exponent_scaling(h, True)
+
+
+def _random_shape_runs(sg, h, ctype, rng):
     for _ in range(24):
         m, n = (int(x) for x in rng.integers(200, 1400, 2))
         k = 64 * int(rng.integers(2, 40))
@@ -438,7 +457,8 @@ def test_presplit_plan_falls_back_where_it_does_not_apply(h):
     assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.tf32)[0] == 8   # BF16 parts packed
     assert sg.query_plan(h, 0, 0, 1024, 1024, 96, sg.tf32)[0] == 5     # odd number of k-tiles
     sg.set_exponent_scaling(h, False)
-    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[0] == 1   # two-accumulator mode
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 1024, sg.fp16)[0] == 8   # two-accumulator mode: 128x128 form
+    assert sg.query_plan(h, 0, 0, 1024, 1024, 96, sg.fp16)[0] == 1     # ... falls back to its large tile
     sg.set_exponent_scaling(h, True)
     sg.force_plan(h, -1, 1)
 
